@@ -1,0 +1,255 @@
+/*
+ * solaris_amd.h -- C ABI of the MI355X (gfx950) force-evaluation and
+ * integration library.
+ *
+ * Every entry point takes plain pointers and sizes; no C++ or GPU runtime
+ * types cross this boundary, and no C++ exception escapes it.  Host arrays
+ * use the reference's array-of-structs layout: 6 doubles per body,
+ * [x, y, z, vx, vy, vz] for states and [vx, vy, vz, ax, ay, az] for
+ * derivatives (Solaris/Acceleration.cpp:258, :274-283).  Inside the context
+ * the state is held on the GPU as structure-of-arrays.
+ *
+ * Return convention: 0 on success, non-zero on failure (the reference
+ * returns 1 for every failure, Solaris/Error.cpp:13-20; here the value also
+ * names the failure class).  sol_last_error() gives the message.
+ *
+ * Threading: one context per host thread; each context owns one HIP stream.
+ * There are no global mutable objects.
+ */
+#ifndef SOLARIS_AMD_H_
+#define SOLARIS_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define SOL_OK              0
+#define SOL_ERROR           1  /* generic failure (reference's "return 1") */
+#define SOL_ERR_ARGUMENT    2  /* bad argument, size or ordering */
+#define SOL_ERR_DEVICE      3  /* HIP runtime failure */
+#define SOL_ERR_UNDERFLOW   4  /* RKF78 step-size underflow, t+h == t
+                                  (Solaris/RungeKuttaFehlberg78.cpp:116-122) */
+#define SOL_ERR_MAXITER     5  /* Dormand-Prince retries exceeded maxIter
+                                  (Solaris/DormandPrince.cpp:158-162) */
+#define SOL_ERR_ALLOCATION  6  /* e.g. nBodies.total == 0
+                                  (Solaris/BodyData.cpp:68-72) */
+#define SOL_ERR_UNSUPPORTED 7  /* integrator or mode not provided
+                                  (e.g. RUNGE_KUTTA56, Solaris/RungeKutta56.h) */
+
+/* ---- enumerations: same numeric values as the reference -------------- */
+/* BodyType, Solaris/Body.h:14-24 */
+enum {
+    SOL_UNDEFINED_BODY_TYPE = 0,
+    SOL_CENTRAL_BODY        = 1,
+    SOL_GIANT_PLANET        = 2,
+    SOL_ROCKY_PLANET        = 3,
+    SOL_PROTO_PLANET        = 4,
+    SOL_SUPER_PLANETESIMAL  = 5,
+    SOL_PLANETESIMAL        = 6,
+    SOL_TEST_PARTICLE       = 7
+};
+/* MigrationType, Solaris/Body.h:35-39 */
+enum { SOL_MIGRATION_NO = 0, SOL_MIGRATION_TYPE_I = 1, SOL_MIGRATION_TYPE_II = 2 };
+/* IntegratorType, Solaris/IntegratorType.h:12-17 */
+enum {
+    SOL_DORMAND_PRINCE          = 0,
+    SOL_RUNGE_KUTTA4            = 1,
+    SOL_RUNGE_KUTTA56           = 2,
+    SOL_RUNGE_KUTTA_FEHLBERG78  = 3
+};
+/* GasDecreaseType, Solaris/GasDecreaseType.h */
+enum { SOL_GAS_CONSTANT = 0, SOL_GAS_LINEAR = 1, SOL_GAS_EXPONENTIAL = 2 };
+/* Arithmetic of the pairwise gravity terms (state update, Kepler term and
+ * error norm are FP64 in both modes). */
+enum { SOL_PRECISION_FP64 = 0, SOL_PRECISION_MIXED = 1 };
+/* Device arrays that sol_get_array() can copy out (BodyData fields,
+ * Solaris/BodyData.h:38-43). */
+enum { SOL_ARRAY_Y0 = 0, SOL_ARRAY_Y = 1, SOL_ARRAY_ACCEL = 2,
+       SOL_ARRAY_YSCALE = 3, SOL_ARRAY_ERROR = 4 };
+
+/* ---- plain-data descriptions ----------------------------------------- */
+/* NBodies, Solaris/NBodies.h.  Bodies must be sorted by type in the order
+ * central, giant, rocky, proto, super-planetesimal, planetesimal, test
+ * particle (Solaris/Simulator.cpp:449-495); the counts define the
+ * interaction ranges. */
+typedef struct sol_nbodies {
+    int32_t central_body;
+    int32_t giant_planet;
+    int32_t rocky_planet;
+    int32_t proto_planet;
+    int32_t super_planetesimal;
+    int32_t planetesimal;
+    int32_t test_particle;
+    int32_t total;
+} sol_nbodies;
+
+/* PowerLaw: c * x^index, Solaris/PowerLaw.cpp:17-20 */
+typedef struct sol_power_law { double c; double index; } sol_power_law;
+
+/* GasComponent fields read by the force model (Solaris/GasComponent.h).
+ * mean_free_path must hold what the reference constructor computed
+ * (Solaris/GasComponent.cpp:30-32); it is not re-derived from density. */
+typedef struct sol_gas_component {
+    double        alpha;
+    double        mean_molecular_weight;
+    double        particle_diameter;
+    int32_t       type;          /* SOL_GAS_* */
+    int32_t       reserved;
+    double        time_scale;
+    double        t0;
+    double        t1;
+    double        inner_edge;
+    double        a;
+    sol_power_law eta;
+    sol_power_law tau;
+    sol_power_law scale_height;
+    sol_power_law density;
+    sol_power_law mean_free_path;
+} sol_gas_component;
+
+/* Per-body parameter arrays (host pointers, each of length total). */
+typedef struct sol_body_params {
+    const double  *mass;
+    const double  *radius;
+    const double  *density;
+    const double  *cd;
+    const double  *gamma_stokes;
+    const double  *gamma_epstein;
+    const double  *mig_stop_at;
+    const int32_t *type;
+    const int32_t *mig_type;
+} sol_body_params;
+
+/* The TimeLine fields the drivers and the caller loop use
+ * (Solaris/TimeLine.h:6-44). */
+typedef struct sol_timeline {
+    double  start;
+    double  length;
+    double  output;
+    double  save;
+    double  time;
+    double  hNext;
+    double  hDid;
+    double  elapsedTime;
+    double  lastSave;
+    double  lastNSteps;
+    int32_t millenium;
+    int32_t reserved;
+} sol_timeline;
+
+/* Counters filled by sol_integrate(). */
+typedef struct sol_run_stats {
+    int64_t accepted_steps;   /* Counter::succededStep (Solaris/Simulator.cpp:154) */
+    int64_t attempts;         /* accepted + rejected attempts */
+    int64_t force_evaluations;
+    double  interactions;     /* sum over evaluations of pair + Kepler terms */
+    double  device_ms;        /* GPU time of the run, HIP events */
+} sol_run_stats;
+
+typedef struct sol_context sol_context;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces the construction of BodyData (Solaris/BodyData.cpp:66-187) and
+ * Acceleration (Solaris/Acceleration.cpp:35-50).  barycentric selects
+ * ComputeBaryCentric vs ComputeAstroCentric (Solaris/Acceleration.cpp:71-78). */
+int sol_create(sol_context **ctx, const sol_nbodies *nbodies,
+               int32_t barycentric, int32_t device);
+int sol_destroy(sol_context *ctx);
+/* Error::_errMsg (Solaris/Error.cpp:13-20); ctx may be NULL for failures of
+ * sol_create.  The string stays valid until the next call on ctx. */
+const char *sol_last_error(const sol_context *ctx);
+
+/* Per-body parameters (BodyData arrays, Solaris/BodyData.h:12-31). */
+int sol_set_bodies(sol_context *ctx, const sol_body_params *params);
+/* Acceleration::nebula (Solaris/Acceleration.h:44); NULL removes the gas. */
+int sol_set_nebula(sol_context *ctx, const sol_gas_component *gas);
+/* Acceleration::evaluateGasDrag / evaluateTypeIMigration /
+ * evaluateTypeIIMigration (Solaris/Acceleration.h:54-56). */
+int sol_set_evaluate(sol_context *ctx, int32_t gas_drag, int32_t type_i,
+                     int32_t type_ii);
+int sol_get_evaluate(const sol_context *ctx, int32_t *gas_drag,
+                     int32_t *type_i, int32_t *type_ii);
+/* Pairwise-gravity arithmetic.  softening (AU) is a Plummer length applied
+ * to pair terms only; 0 reproduces the reference. */
+int sol_set_precision(sol_context *ctx, int32_t precision, double softening);
+/* Source ranges [1, end) of the two astrocentric sink classes (bodies of
+ * type <= ProtoPlanet, and all others).  -1 keeps the reference's
+ * type-based ranges (Solaris/Acceleration.cpp:285-289).  No counterpart in
+ * the reference: it expresses mutually attracting planetesimals. */
+int sol_set_source_ranges(sol_context *ctx, int32_t end_massive_sinks,
+                          int32_t end_other_sinks);
+
+/* ---- force model ------------------------------------------------------ */
+/* Acceleration::Compute(double t, double *y, double *totalAccel)
+ * (Solaris/Acceleration.h:19, Solaris/Acceleration.cpp:60-81).
+ * y and dydt are host arrays of 6*total doubles. */
+int sol_compute(sol_context *ctx, double t, const double *y, double *dydt);
+/* Side outputs of the last force evaluation: Acceleration::rm3
+ * (Solaris/Acceleration.h:52), BodyData::indexOfNN / distanceOfNN and the
+ * mutated BodyData::migType (Solaris/Acceleration.cpp:253-264, :439-443).
+ * Any pointer may be NULL. */
+int sol_get_side_outputs(sol_context *ctx, double *rm3, int32_t *index_nn,
+                         double *distance_nn, int32_t *mig_type);
+/* Overwrites the device copy of BodyData::migType. */
+int sol_set_mig_type(sol_context *ctx, const int32_t *mig_type);
+
+/* ---- integrators ------------------------------------------------------ */
+/* Upload BodyData::y0 (6*total doubles, host). */
+int sol_set_state(sol_context *ctx, const double *y0);
+/* Copy a device array (SOL_ARRAY_*) of 6*total doubles to the host. */
+int sol_get_array(sol_context *ctx, int32_t which, double *out);
+/* Upload BodyData::accel (the derivative at y0 that Step() reads as f0). */
+int sol_set_accel(sol_context *ctx, const double *accel);
+
+/* One Driver call on the device-resident state:
+ *   RungeKuttaFehlberg78::Driver (Solaris/RungeKuttaFehlberg78.cpp:66-140),
+ *   RungeKutta4::Driver          (Solaris/RungeKutta4.cpp:20-56),
+ *   DormandPrince::Driver        (Solaris/DormandPrince.cpp:126-170).
+ * Reads tl->time and tl->hNext; writes tl->hDid, tl->time, tl->hNext.
+ * epsilon is the integrator's 10^accuracy.  After success the device Y0
+ * holds the new state and Y the previous one (the reference's pointer
+ * swap, Solaris/RungeKuttaFehlberg78.cpp:132).  body_h receives
+ * BodyData::h after the call. */
+int sol_driver(sol_context *ctx, int32_t integrator, double epsilon,
+               sol_timeline *tl, double *body_h);
+/* One Step call from device Y0 and ACCEL at (time, h) into Y and ERROR:
+ *   RungeKuttaFehlberg78::Step (Solaris/RungeKuttaFehlberg78.cpp:147-250),
+ *   RungeKutta4::Step          (Solaris/RungeKutta4.cpp:58-130),
+ *   DormandPrince::Step2       (Solaris/DormandPrince.cpp:246-491). */
+int sol_step(sol_context *ctx, int32_t integrator, double time, double h);
+/* RungeKuttaFehlberg78::GetErrorMax (Solaris/RungeKuttaFehlberg78.cpp:252-262)
+ * over the device ERROR and YSCALE arrays, divided by epsilon; with
+ * integrator == SOL_DORMAND_PRINCE, DormandPrince::GetErrorMax
+ * (Solaris/DormandPrince.cpp:493-503) over ERROR alone. */
+int sol_error_max(sol_context *ctx, int32_t integrator, double epsilon,
+                  double *error_max);
+
+/* The Simulator::Integrate loop (Solaris/Simulator.cpp:121-178) without
+ * events and file output: Driver calls, the step clipping of
+ * Simulator::DecisionMaking (Solaris/Simulator.cpp:181-248) and the
+ * |y| < 1e-50 flush every 100 steps (:159-162), until the end of the
+ * time line or max_steps accepted steps (max_steps <= 0: no limit).
+ * stats may be NULL. */
+int sol_integrate(sol_context *ctx, int32_t integrator, double epsilon,
+                  sol_timeline *tl, int64_t max_steps, sol_run_stats *stats);
+
+/* Calculate::Integrals on device Y0 (Solaris/Calculate.cpp:43-63):
+ * out[16] = M, barycentre r and v, |r_bc|, |v_bc|, angular momentum and its
+ * length, kinetic, potential and total energy. */
+int sol_integrals(sol_context *ctx, double *out16);
+
+/* Interactions per force evaluation under the context's source ranges:
+ * sum over sinks i >= 1 of (1 + |S(i) \ {i}|). */
+int sol_interactions_per_evaluation(const sol_context *ctx, double *count);
+
+/* Library version string. */
+const char *sol_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOLARIS_AMD_H_ */

@@ -1,0 +1,1005 @@
+/*
+ * cpu_ref.c -- the oracle: FP64 restatement of the Solaris hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see cpu_ref.h).  Loop order, evaluation order
+ * and the reference's quirks are kept on purpose, so that this file agrees
+ * with the compiled reference to the last few ulps:
+ *   - Kepler term stored first, pair sum added last, j ascending
+ *     (Solaris/Acceleration.cpp:268-326);
+ *   - type-based source ranges (Solaris/Acceleration.cpp:285-289);
+ *   - every RKF78 stage evaluated at time t (RungeKuttaFehlberg78.cpp:170-234);
+ *   - migration evaluated once per Driver call and the cached value re-added
+ *     (RungeKuttaFehlberg78.cpp:77-95, Acceleration.cpp:200-242);
+ *   - yscale from the first trial h only (RungeKuttaFehlberg78.cpp:87-89);
+ *   - the swapped Temperature_CMU arguments inside MeanThermalSpeed_CMU
+ *     (Solaris/GasComponent.cpp:241);
+ *   - |x| of doubles is fabs (MSVC semantics of abs(double),
+ *     Solaris/Ephemeris.cpp:30, Solaris/Acceleration.cpp:770).
+ * Deliberate, documented deviations from undefined behaviour:
+ *   - the Type I cache is sized for rocky + proto planets (the reference
+ *     sizes it by proto planets only, Solaris/Acceleration.cpp:108-111);
+ *   - in barycentric mode, once the gas has decayed below 1e-5 the gas
+ *     terms are switched off for good (the reference deletes the nebula and
+ *     keeps using the pointer, Solaris/Acceleration.cpp:151-160).
+ * Build with -ffp-contract=off so that no FMA contraction changes rounding.
+ */
+#include "cpu_ref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- Solaris/Constants.h, same defining expressions ------------------ */
+#define C_PI               3.14159265358979323846
+#define C_SQRT_TWO_PI      2.50662827463100024161
+#define C_BOLTZMAN_SI      1.3806488e-23
+#define C_PROTON_MASS_SI   1.672621777e-27
+#define C_GAUSS            1.720209895e-2
+#define C_GAUSS2           2.959122082855911025e-4
+#define C_SOLAR_TO_KG      1.98911e30
+#define C_SOLAR_TO_GRAM    (1.0e3 * C_SOLAR_TO_KG)
+#define C_KG_TO_SOLAR      (1.0 / C_SOLAR_TO_KG)
+#define C_GRAM_TO_SOLAR    (1.0 / C_SOLAR_TO_GRAM)
+#define C_AU_TO_METER      1.495978707e11
+#define C_METER_TO_AU      (1.0 / C_AU_TO_METER)
+#define C_AU_TO_SOLAR_RADIUS 215.094
+#define C_SOLAR_RADIUS_TO_AU (1.0 / C_AU_TO_SOLAR_RADIUS)
+#define C_YEAR_TO_DAY      365.25
+#define C_DAY_TO_SECOND    86400.0
+#define C_SECOND_TO_DAY    (1.0 / C_DAY_TO_SECOND)
+#define C_GCM3_TO_SOLAR_AU3 (C_GRAM_TO_SOLAR / ((1.0e-2*C_METER_TO_AU)*(1.0e-2*C_METER_TO_AU)*(1.0e-2*C_METER_TO_AU)))
+#define C_BOLTZMAN_CMU     (C_BOLTZMAN_SI * (C_KG_TO_SOLAR * ((C_METER_TO_AU)*(C_METER_TO_AU))) / ((C_SECOND_TO_DAY)*(C_SECOND_TO_DAY)))
+#define C_PROTON_MASS_CMU  (C_PROTON_MASS_SI * C_KG_TO_SOLAR)
+#define C_BOLTZMAN_PROTON_MASS_CMU (C_BOLTZMAN_CMU / C_PROTON_MASS_CMU)
+#define C_PROTON_MASS_BOLTZMAN_CMU (1.0 / C_BOLTZMAN_PROTON_MASS_CMU)
+#define C_SMALLEST_NUMBER  1.0e-50
+#define C_CHECK_FOR_SM     100
+
+#define SQR(a)   ((a)*(a))
+#define CUBE(a)  ((a)*(a)*(a))
+#define FORTH(a) ((a)*(a)*(a)*(a))
+#define FIFTH(a) ((a)*(a)*(a)*(a)*(a))
+
+double cr_constant(const char *name)
+{
+    static const struct { const char *n; double v; } tab[] = {
+        {"Pi", C_PI}, {"SqrtTwoPi", C_SQRT_TWO_PI}, {"Gauss", C_GAUSS},
+        {"Gauss2", C_GAUSS2}, {"GramPerCm3ToSolarPerAu3", C_GCM3_TO_SOLAR_AU3},
+        {"Boltzman_CMU", C_BOLTZMAN_CMU}, {"ProtonMass_CMU", C_PROTON_MASS_CMU},
+        {"BoltzmanProtonMass_CMU", C_BOLTZMAN_PROTON_MASS_CMU},
+        {"SolarRadiusToAu", C_SOLAR_RADIUS_TO_AU}, {"MeterToAu", C_METER_TO_AU},
+        {"SecondToDay", C_SECOND_TO_DAY}, {"YearToDay", C_YEAR_TO_DAY},
+        {"AuToMeter", C_AU_TO_METER},
+    };
+    for (size_t i = 0; i < sizeof(tab)/sizeof(tab[0]); i++)
+        if (strcmp(tab[i].n, name) == 0) return tab[i].v;
+    return NAN;
+}
+
+/* ---- system ---------------------------------------------------------- */
+struct cr_system {
+    sol_nbodies nb;
+    int32_t barycentric;
+    int32_t n;
+    double *mass, *radius, *density, *cd, *gamma_stokes, *gamma_epstein, *mig_stop_at;
+    int32_t *type, *mig_type;
+    double *rm3; int32_t *index_nn; double *distance_nn;
+    int32_t has_gas, gas_dead;
+    sol_gas_component gas;
+    double *acc_drag, *acc_mig1, *acc_mig2;   /* caches, 3 per body */
+    int32_t eval_drag, eval_mig1, eval_mig2;
+    double soft2;
+    int32_t end_massive, end_other;           /* -1: reference ranges */
+    /* integrator state (BodyData, Solaris/BodyData.h:38-43) */
+    double *y0, *y, *accel, *yscale, *error;
+    double time, h;
+    double evals;                             /* force evaluations so far */
+};
+
+static int n_massive(const sol_nbodies *nb)
+{   /* Solaris/NBodies.cpp:21-24 */
+    return nb->central_body + nb->giant_planet + nb->rocky_planet + nb->proto_planet;
+}
+
+static int n_pl_spl(const sol_nbodies *nb)
+{   /* Solaris/NBodies.cpp:26-29 */
+    return nb->super_planetesimal + nb->planetesimal;
+}
+
+cr_system *cr_create(const sol_nbodies *nb, int32_t barycentric)
+{
+    if (nb->total <= 0) return NULL;
+    cr_system *s = (cr_system *)calloc(1, sizeof(cr_system));
+    int n = nb->total;
+    s->nb = *nb; s->n = n; s->barycentric = barycentric;
+    s->mass = calloc(n, 8); s->radius = calloc(n, 8); s->density = calloc(n, 8);
+    s->cd = calloc(n, 8); s->gamma_stokes = calloc(n, 8); s->gamma_epstein = calloc(n, 8);
+    s->mig_stop_at = calloc(n, 8); s->type = calloc(n, 4); s->mig_type = calloc(n, 4);
+    s->rm3 = calloc(n, 8); s->index_nn = calloc(n, 4); s->distance_nn = calloc(n, 8);
+    s->acc_drag = calloc(3*(size_t)n, 8); s->acc_mig1 = calloc(3*(size_t)n, 8);
+    s->acc_mig2 = calloc(3*(size_t)n, 8);
+    s->y0 = calloc(6*(size_t)n, 8); s->y = calloc(6*(size_t)n, 8);
+    s->accel = calloc(6*(size_t)n, 8); s->yscale = calloc(6*(size_t)n, 8);
+    s->error = calloc(6*(size_t)n, 8);
+    /* Acceleration ctor (Solaris/Acceleration.cpp:35-50) */
+    s->eval_drag = s->eval_mig1 = s->eval_mig2 = 1;
+    s->end_massive = s->end_other = -1;
+    return s;
+}
+
+void cr_destroy(cr_system *s)
+{
+    if (!s) return;
+    free(s->mass); free(s->radius); free(s->density); free(s->cd);
+    free(s->gamma_stokes); free(s->gamma_epstein); free(s->mig_stop_at);
+    free(s->type); free(s->mig_type); free(s->rm3); free(s->index_nn);
+    free(s->distance_nn); free(s->acc_drag); free(s->acc_mig1); free(s->acc_mig2);
+    free(s->y0); free(s->y); free(s->accel); free(s->yscale); free(s->error);
+    free(s);
+}
+
+void cr_set_bodies(cr_system *s, const sol_body_params *p)
+{
+    size_t n = (size_t)s->n;
+#define CPY(dst, src, sz) do { if (src) memcpy(dst, src, n*(sz)); else memset(dst, 0, n*(sz)); } while (0)
+    CPY(s->mass, p->mass, 8); CPY(s->radius, p->radius, 8); CPY(s->density, p->density, 8);
+    CPY(s->cd, p->cd, 8); CPY(s->gamma_stokes, p->gamma_stokes, 8);
+    CPY(s->gamma_epstein, p->gamma_epstein, 8); CPY(s->mig_stop_at, p->mig_stop_at, 8);
+    CPY(s->type, p->type, 4); CPY(s->mig_type, p->mig_type, 4);
+#undef CPY
+}
+
+void cr_set_nebula(cr_system *s, const sol_gas_component *g)
+{
+    if (g) { s->gas = *g; s->has_gas = 1; } else s->has_gas = 0;
+    s->gas_dead = 0;
+}
+
+void cr_set_evaluate(cr_system *s, int32_t d, int32_t m1, int32_t m2)
+{ s->eval_drag = d; s->eval_mig1 = m1; s->eval_mig2 = m2; }
+
+void cr_get_evaluate(const cr_system *s, int32_t *d, int32_t *m1, int32_t *m2)
+{ if (d) *d = s->eval_drag; if (m1) *m1 = s->eval_mig1; if (m2) *m2 = s->eval_mig2; }
+
+void cr_set_softening(cr_system *s, double softening) { s->soft2 = softening*softening; }
+
+void cr_set_source_ranges(cr_system *s, int32_t a, int32_t b) { s->end_massive = a; s->end_other = b; }
+
+void cr_set_mig_type(cr_system *s, const int32_t *m) { memcpy(s->mig_type, m, (size_t)s->n*4); }
+
+void cr_get_side_outputs(const cr_system *s, double *rm3, int32_t *nn, double *dist, int32_t *mig)
+{
+    size_t n = (size_t)s->n;
+    if (rm3) memcpy(rm3, s->rm3, n*8);
+    if (nn) memcpy(nn, s->index_nn, n*4);
+    if (dist) memcpy(dist, s->distance_nn, n*8);
+    if (mig) memcpy(mig, s->mig_type, n*4);
+}
+
+/* Source range end for an astrocentric sink (Solaris/Acceleration.cpp:285-289). */
+static int src_end_ac(const cr_system *s, int i)
+{
+    int nm = n_massive(&s->nb);
+    if (s->type[i] <= SOL_PROTO_PLANET)
+        return s->end_massive >= 0 ? s->end_massive : nm + s->nb.super_planetesimal;
+    return s->end_other >= 0 ? s->end_other : nm;
+}
+
+double cr_interactions_per_evaluation(const cr_system *s)
+{
+    double total = 0.0;
+    int nm = n_massive(&s->nb);
+    for (int i = 1; i < s->n; i++) {
+        if (s->barycentric) {
+            /* GravityBC: massive sinks see [0,nm)\{i}; others see [0,nm) */
+            total += (i < nm) ? (double)(nm - 1) : (double)nm;
+        } else {
+            int e = src_end_ac(s, i);
+            double cnt = (e > 1) ? (double)(e - 1) : 0.0;
+            if (i >= 1 && i < e) cnt -= 1.0;
+            total += 1.0 + cnt;
+        }
+    }
+    if (s->barycentric) total += (double)(nm - 1);   /* the star as a sink */
+    return total;
+}
+
+/* ---- gravity --------------------------------------------------------- */
+/* Acceleration::GravityAC, Solaris/Acceleration.cpp:248-329. */
+static void gravity_ac(cr_system *s, const double *y, double *a)
+{
+    const int n = s->n;
+    s->index_nn[0] = -1;
+    s->distance_nn[0] = 0.0;
+    for (int i = 1; i < n; i++) {                       /* :256-264 */
+        const double *p = y + 6*i;
+        double r2 = SQR(p[0]) + SQR(p[1]) + SQR(p[2]);
+        double r = sqrt(r2);
+        s->rm3[i] = 1.0 / (r2 * r);
+        s->index_nn[i] = -1;
+        s->distance_nn[i] = 0.0;
+    }
+    a[0] = a[1] = a[2] = a[3] = a[4] = a[5] = 0.0;      /* :266 */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int i = 1; i < n; i++) {                       /* :268-326 */
+        const int i0 = 6*i;
+        double rmin = 1.0e10;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        const double mu = C_GAUSS2 * (s->mass[0] + s->mass[i]);
+        a[i0 + 0] = y[i0 + 3];
+        a[i0 + 1] = y[i0 + 4];
+        a[i0 + 2] = y[i0 + 5];
+        a[i0 + 3] = -mu * s->rm3[i] * y[i0 + 0];         /* Kepler term first */
+        a[i0 + 4] = -mu * s->rm3[i] * y[i0 + 1];
+        a[i0 + 5] = -mu * s->rm3[i] * y[i0 + 2];
+        const int end = src_end_ac(s, i);
+        for (int j = 1; j < end; j++) {                 /* j ascending, j != i */
+            if (j == i) continue;
+            const int j0 = 6*j;
+            double xij = y[j0 + 0] - y[i0 + 0];
+            double yij = y[j0 + 1] - y[i0 + 1];
+            double zij = y[j0 + 2] - y[i0 + 2];
+            double rij2 = SQR(xij) + SQR(yij) + SQR(zij);
+            double rij = sqrt(rij2);
+            double rs2 = rij2 + s->soft2;               /* soft2 == 0: reference */
+            double rijm3 = (s->soft2 == 0.0) ? 1.0 / (rij2 * rij) : 1.0 / (rs2 * sqrt(rs2));
+            if (rij < rmin) {
+                rmin = rij;
+                s->index_nn[i] = j;
+                s->distance_nn[i] = rij;
+            }
+            double gmj = C_GAUSS2 * s->mass[j];
+            ax += gmj * (xij * rijm3 - y[j0 + 0] * s->rm3[j]);
+            ay += gmj * (yij * rijm3 - y[j0 + 1] * s->rm3[j]);
+            az += gmj * (zij * rijm3 - y[j0 + 2] * s->rm3[j]);
+        }
+        a[i0 + 3] += ax;                                 /* pair sum added last */
+        a[i0 + 4] += ay;
+        a[i0 + 5] += az;
+    }
+}
+
+/* GravityBC_SelfInteracting and _NonSelfInteracting,
+ * Solaris/Acceleration.cpp:541-634: direct sum, the star is an ordinary
+ * source, j descending, K^2 applied at the end.  rm3 is not touched. */
+static void gravity_bc(cr_system *s, const double *y, double *a)
+{
+    const int n = s->n, nm = n_massive(&s->nb);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int i = 0; i < n; i++) {
+        const int i0 = 6*i;
+        double rmin = 1.0e10;
+        s->index_nn[i] = -1;
+        s->distance_nn[i] = 0.0;
+        a[i0 + 0] = y[i0 + 3];
+        a[i0 + 1] = y[i0 + 4];
+        a[i0 + 2] = y[i0 + 5];
+        a[i0 + 3] = a[i0 + 4] = a[i0 + 5] = 0.0;
+        for (int j = nm - 1; j >= 0; j--) {
+            if (j == i) continue;
+            const int j0 = 6*j;
+            double dx = y[j0 + 0] - y[i0 + 0];
+            double dy = y[j0 + 1] - y[i0 + 1];
+            double dz = y[j0 + 2] - y[i0 + 2];
+            double rij2 = SQR(dx) + SQR(dy) + SQR(dz);
+            double rij = sqrt(rij2);
+            if (rij < rmin) {
+                rmin = rij;
+                s->index_nn[i] = j;
+                s->distance_nn[i] = rij;
+            }
+            double rs2 = rij2 + s->soft2;
+            double c = (s->soft2 == 0.0) ? s->mass[j] * 1.0 / (rij2 * rij)
+                                         : s->mass[j] * 1.0 / (rs2 * sqrt(rs2));
+            a[i0 + 3] += c * dx;
+            a[i0 + 4] += c * dy;
+            a[i0 + 5] += c * dz;
+        }
+        a[i0 + 3] *= C_GAUSS2;
+        a[i0 + 4] *= C_GAUSS2;
+        a[i0 + 5] *= C_GAUSS2;
+    }
+}
+
+/* ---- gas helpers (Solaris/GasComponent.cpp, Solaris/PowerLaw.cpp) ---- */
+static double pl_eval(sol_power_law p, double x) { return p.c * pow(x, p.index); }
+
+static double reduction_factor(const sol_gas_component *g, double t)
+{   /* GasComponent::ReductionFactor, Solaris/GasComponent.cpp:36-61 */
+    switch (g->type) {
+    case SOL_GAS_CONSTANT: return 1.0;
+    case SOL_GAS_LINEAR:
+        if (t <= g->t0) return 1.0;
+        else if (t > g->t0 && t <= g->t1) return 1.0 - (t - g->t0)/(g->t1 - g->t0);
+        else return 0.0;
+    case SOL_GAS_EXPONENTIAL: return exp(-t/g->time_scale);
+    default: return 1.0;
+    }
+}
+
+static double midplane_density(const sol_gas_component *g, double r)
+{   /* Solaris/GasComponent.cpp:63-69 */
+    double a1 = pl_eval(g->density, r);
+    double a2 = pl_eval(g->scale_height, r);
+    return a1*a2*C_SQRT_TWO_PI;
+}
+
+void cr_circular_velocity(double mu, const double *rv, double *out)
+{   /* GasComponent::circular_velocity, Solaris/GasComponent.cpp:97-126 */
+    double x = rv[0], y = rv[1];
+    double r = sqrt(SQR(x) + SQR(y));
+    double vc = sqrt(mu/r);
+    double vx = 0.0, vy = 0.0, p;
+    if (x == 0.0 && y == 0.0) {
+    } else if (y == 0.0) {
+        vy = x > 0.0 ? vc : -vc;
+    } else if (x == 0.0) {
+        vx = y > 0.0 ? -vc : vc;
+    } else if (x >= y) {
+        p = y / x;
+        vy = x >= 0 ? vc/sqrt(1.0 + SQR(p)) : -vc/sqrt(1.0 + SQR(p));
+        vx = -vy*p;
+    } else {
+        p = x / y;
+        vx = y >= 0 ? -vc/sqrt(1.0 + SQR(p)) : vc/sqrt(1.0 + SQR(p));
+        vy = -vx*p;
+    }
+    out[0] = vx; out[1] = vy; out[2] = 0.0;
+}
+
+void cr_gas_velocity(const sol_gas_component *g, double mu, const double *rv, double *out)
+{   /* GasComponent::gas_velocity, Solaris/GasComponent.cpp:128-138 */
+    cr_circular_velocity(mu, rv, out);
+    double r = sqrt(SQR(rv[0]) + SQR(rv[1]));
+    double v = sqrt(1.0 - 2.0*pl_eval(g->eta, r));
+    out[0] *= v;
+    out[1] *= v;
+}
+
+double cr_gas_density_at(const sol_gas_component *g, const double *rv)
+{   /* GasComponent::gas_density_at, Solaris/GasComponent.cpp:141-157 */
+    double r = sqrt(SQR(rv[0]) + SQR(rv[1]));
+    double h = pl_eval(g->scale_height, r);
+    double arg = SQR(rv[2]/h);
+    if (g->inner_edge < r) {
+        return pl_eval(g->density, r) * exp(-arg);
+    } else {
+        double a = g->density.c * pow(g->inner_edge, g->density.index - 4.0);
+        return a * SQR(SQR(r)) * exp(-arg);
+    }
+}
+
+double cr_temperature_cmu(const sol_gas_component *g, double mc, double r)
+{   /* GasComponent::Temperature_CMU, Solaris/GasComponent.cpp:221-229 */
+    double ctp = C_GAUSS2 * C_PROTON_MASS_BOLTZMAN_CMU;
+    double pt = 2.0 * g->scale_height.index - 3.0;
+    double ct = SQR(g->scale_height.c) * mc * g->mean_molecular_weight * ctp;
+    return ct * pow(r, pt);
+}
+
+double cr_mean_thermal_speed_cmu(const sol_gas_component *g, double mc, double r)
+{   /* GasComponent::MeanThermalSpeed_CMU, Solaris/GasComponent.cpp:238-243:
+     * the temperature is called with (r, mC), i.e. swapped. */
+    double cvth = sqrt((8.0 * C_BOLTZMAN_CMU)/(C_PI * g->mean_molecular_weight * C_PROTON_MASS_CMU));
+    return cvth * sqrt(cr_temperature_cmu(g, r, mc));
+}
+
+void cr_default_gas(sol_gas_component *g)
+{   /* GasComponent::GasComponent, Solaris/GasComponent.cpp:9-34 */
+    memset(g, 0, sizeof(*g));
+    g->inner_edge = 10.0 * C_SOLAR_RADIUS_TO_AU;
+    g->alpha = 2.0e-3;
+    g->mean_molecular_weight = 2.3;
+    g->particle_diameter = 3.0e-10;
+    g->type = SOL_GAS_CONSTANT;
+    g->eta.c = 0.0019; g->eta.index = 0.5;
+    g->tau.c = 2.0/3.0; g->tau.index = 2.0;
+    g->scale_height.c = 0.02; g->scale_height.index = 1.25;
+    g->density.c = 1.4e-9 * C_GCM3_TO_SOLAR_AU3; g->density.index = -2.75;
+    g->a = pl_eval(g->density, g->inner_edge) / SQR(SQR(g->inner_edge));
+    double clambda = g->mean_molecular_weight * C_PROTON_MASS_CMU /
+        (sqrt(2.0) * C_PI * SQR(g->particle_diameter * C_METER_TO_AU) * g->density.c);
+    g->mean_free_path.c = clambda;
+    g->mean_free_path.index = -g->density.index;
+}
+
+/* Acceleration::GasDragAC, Solaris/Acceleration.cpp:331-422 (= GasDragBC). */
+static void gas_drag(cr_system *s, double t, const double *y, double *acc)
+{
+    const sol_gas_component *g = &s->gas;
+    double factor = reduction_factor(g, t);
+    int lower = n_massive(&s->nb);
+    int upper = lower + s->nb.planetesimal + s->nb.super_planetesimal;
+    for (int i = lower; i < upper; i++) {
+        const double *p = y + 6*i;
+        double *o = acc + 3*(i - lower);
+        double r = sqrt(SQR(p[0]) + SQR(p[1]) + SQR(p[2]));
+        double vg[3];
+        cr_gas_velocity(g, C_GAUSS2*s->mass[0], p, vg);
+        double ux = p[3] - vg[0], uy = p[4] - vg[1], uz = p[5] - vg[2];
+        double rho = factor * cr_gas_density_at(g, p);
+        double lambda = pl_eval(g->mean_free_path, r);
+        double C;
+        if (s->radius[i] <= 0.1 * lambda) {                      /* Epstein */
+            double vth = cr_mean_thermal_speed_cmu(g, s->mass[0], r);
+            C = s->gamma_epstein[i] * vth * rho;
+        } else if (s->radius[i] >= 10.0 * lambda) {              /* Stokes */
+            double ul = sqrt(ux*ux + uy*uy + uz*uz);
+            C = s->gamma_stokes[i] * ul * rho;
+        } else {                                                  /* transition */
+            double l1 = 0.1 * lambda, l2 = 10.0 * lambda;
+            double ge = 1.0/(s->density[i] * l1);
+            double gs = 3.0/8.0 * s->cd[i] / (s->density[i] * l2);
+            double vth = cr_mean_thermal_speed_cmu(g, s->mass[0], r);
+            double K = gs * sqrt(ux*ux + uy*uy + uz*uz) / (ge * vth);
+            double eta = l2 / l1;
+            double kappa = log10(K) / log10(eta);
+            double gamma = ge * vth * pow(l1, -kappa);
+            C = gamma * pow(s->radius[i], kappa) * rho;
+        }
+        o[0] = -C * ux;
+        o[1] = -C * uy;
+        o[2] = -C * uz;
+    }
+}
+
+/* Ephemeris::CalculateOrbitalElement(mu, phase, &a, &e),
+ * Solaris/Ephemeris.cpp:10-41, with fabs for abs. Returns 1 (and leaves
+ * a, e untouched) for an unbound orbit. */
+int cr_orbital_a_e(double mu, const double *rv, double *a, double *e)
+{
+    double v2 = SQR(rv[3]) + SQR(rv[4]) + SQR(rv[5]);
+    double rl = sqrt(SQR(rv[0]) + SQR(rv[1]) + SQR(rv[2]));
+    double h = v2 / 2.0 + (-mu / rl);
+    if (h >= 0.0) return 1;
+    double cx = rv[1]*rv[5] - rv[2]*rv[4];
+    double cy = rv[2]*rv[3] - rv[0]*rv[5];
+    double cz = rv[0]*rv[4] - rv[1]*rv[3];
+    double c2 = SQR(cx) + SQR(cy) + SQR(cz);
+    double e2 = 1.0 + 2.0 * c2 * h / (mu * mu);
+    if (fabs(e2) < 1.0e-14) e2 = 0.0;
+    *e = sqrt(e2);
+    *a = -mu / (2.0 * h);
+    return 0;
+}
+
+static double type1_migration_time(const sol_gas_component *g, double C, double O,
+                                   double ar, double er, double h)
+{   /* Acceleration::TypeIMigrationTime, Solaris/Acceleration.cpp:766-777 */
+    double Cm = 2.0/(2.7 + 1.1*fabs(g->density.index))/O;
+    double er1 = er/(1.3*h);
+    double er2 = er/(1.1*h);
+    double frac = (1.0 + FIFTH(er1))/(1.0 - FORTH(er2));
+    return Cm*C*SQR(ar)*frac;
+}
+
+static double type1_ecc_damping_time(double C, double O, double ar, double er, double h)
+{   /* Solaris/Acceleration.cpp:779-789 */
+    const double Q = 0.78;
+    double Ce = 0.1/(Q*O);
+    double frac = 1.0 + 0.25*CUBE(er/h);
+    return Ce*C*FORTH(ar)*frac;
+}
+
+static double tau_nu(const sol_gas_component *g, double r, double O)
+{   /* Acceleration::TauNu, Solaris/Acceleration.cpp:831-849 */
+    double h = pl_eval(g->scale_height, r);
+    if (g->tau.index == 2)
+        return g->tau.c * SQR(r/h) / (g->alpha*O);
+    return g->tau.c * pow(r/h, g->tau.index) / (g->alpha*O);
+}
+
+/* MigrationTypeIAC / MigrationTypeIBC, Solaris/Acceleration.cpp:424-485,
+ * :645-713 (identical formulas: TransformToAC is an empty stub). */
+static void migration_type1(cr_system *s, double t, const double *y, double *acc)
+{
+    const sol_gas_component *g = &s->gas;
+    double factor = reduction_factor(g, t);
+    int lower = s->nb.central_body + s->nb.giant_planet;
+    int upper = n_massive(&s->nb);
+    for (int i = lower; i < upper; i++) {
+        if (s->mig_type[i] != SOL_MIGRATION_TYPE_I) continue;
+        const double *p = y + 6*i;
+        double *o = acc + 3*(i - lower);
+        double r2 = SQR(p[0]) + SQR(p[1]) + SQR(p[2]);
+        double r = sqrt(r2);
+        if (r <= s->mig_stop_at[i]) {
+            o[0] = o[1] = o[2] = 0.0;
+            s->mig_type[i] = SOL_MIGRATION_NO;
+            continue;
+        }
+        double m = s->mass[i], mc = s->mass[0];
+        double a = 0.0, e = 0.0;
+        double mu = C_GAUSS2*(mc + m);
+        cr_orbital_a_e(mu, p, &a, &e);                  /* error ignored, :452 */
+        double O = C_GAUSS * sqrt((mc + m)/CUBE(a));
+        double C = SQR(mc)/(m*midplane_density(g, r)*a*a);
+        double h = pl_eval(g->scale_height, r);
+        double ar = h/r;
+        double er = e*r;
+        double tm = 0.0;
+        if (e < 1.1*h/r) {
+            tm = type1_migration_time(g, C, O, ar, er, h);
+            tm = 1.0/tm;
+        }
+        double te = type1_ecc_damping_time(C, O, ar, er, h);
+        double ti = te;
+        double vr = p[0]*p[3] + p[1]*p[4] + p[2]*p[5];
+        te = 2.0*vr/(r2*te);
+        ti = 2.0/ti;
+        o[0] = -factor*(tm*p[3] + te*p[0]);
+        o[1] = -factor*(tm*p[4] + te*p[1]);
+        o[2] = -factor*(tm*p[5] + te*p[2] + ti*p[5]);
+    }
+}
+
+/* MigrationTypeIIAC / MigrationTypeIIBC, Solaris/Acceleration.cpp:487-528,
+ * :716-764; the BC form uses c0 = TauNu instead of 1/TauNu (:754). */
+static void migration_type2(cr_system *s, double t, const double *y, double *acc)
+{
+    const sol_gas_component *g = &s->gas;
+    double factor = reduction_factor(g, t);
+    int lower = s->nb.central_body;
+    int upper = s->nb.central_body + s->nb.giant_planet;
+    for (int i = lower; i < upper; i++) {
+        if (s->mig_type[i] != SOL_MIGRATION_TYPE_II) continue;
+        const double *p = y + 6*i;
+        double *o = acc + 3*(i - lower);
+        double r2 = SQR(p[0]) + SQR(p[1]) + SQR(p[2]);
+        double r = sqrt(r2);
+        if (r <= s->mig_stop_at[i]) {
+            s->mig_type[i] = SOL_MIGRATION_NO;
+            o[0] = o[1] = o[2] = 0.0;
+            continue;
+        }
+        double m = s->mass[i], mc = s->mass[0];
+        double a = 0.0, e = 0.0;
+        double mu = C_GAUSS2*(mc + m);
+        cr_orbital_a_e(mu, p, &a, &e);
+        double O = C_GAUSS * sqrt((mc + m)/CUBE(a));
+        double c0 = s->barycentric ? tau_nu(g, r, O) : 1.0/tau_nu(g, r, O);
+        double vr = p[3]*p[0] + p[4]*p[1] + p[5]*p[2];
+        double c1 = vr/r2;
+        o[0] = -factor*(c0 * (0.5*p[3] + 50 * (c1*p[0])));
+        o[1] = -factor*(c0 * (0.5*p[4] + 50 * (c1*p[1])));
+        o[2] = -factor*(c0 * (0.5*p[5] + 50 * (c1*p[2]) + p[5]));
+    }
+}
+
+/* Acceleration::Compute -> ComputeAstroCentric / ComputeBaryCentric,
+ * Solaris/Acceleration.cpp:60-246. */
+int cr_compute(cr_system *s, double t, const double *y, double *f)
+{
+    if (s->barycentric) gravity_bc(s, y, f);
+    else gravity_ac(s, y, f);
+    s->evals += 1.0;
+    if (!s->has_gas || s->gas_dead) return 0;
+
+    int nm = n_massive(&s->nb);
+    int npl = n_pl_spl(&s->nb);
+    if (s->eval_drag && npl > 0) gas_drag(s, t, y, s->acc_drag);
+    for (int i = nm; i < nm + npl; i++) {
+        const double *c = s->acc_drag + 3*(i - nm);
+        f[6*i + 3] += c[0]; f[6*i + 4] += c[1]; f[6*i + 5] += c[2];
+    }
+    if (s->eval_mig1 && s->nb.proto_planet > 0) migration_type1(s, t, y, s->acc_mig1);
+    int lo = s->nb.central_body + s->nb.giant_planet;
+    for (int i = lo; i < nm; i++) {
+        if (s->mig_type[i] == SOL_MIGRATION_NO) continue;
+        const double *c = s->acc_mig1 + 3*(i - lo);
+        f[6*i + 3] += c[0]; f[6*i + 4] += c[1]; f[6*i + 5] += c[2];
+    }
+    if (s->eval_mig2 && s->nb.giant_planet > 0) migration_type2(s, t, y, s->acc_mig2);
+    for (int i = s->nb.central_body; i < s->nb.central_body + s->nb.giant_planet; i++) {
+        if (s->mig_type[i] == SOL_MIGRATION_NO) continue;
+        const double *c = s->acc_mig2 + 3*(i - s->nb.central_body);
+        f[6*i + 3] += c[0]; f[6*i + 4] += c[1]; f[6*i + 5] += c[2];
+    }
+    if (s->barycentric && reduction_factor(&s->gas, t) < 1.0e-5) {
+        /* Solaris/Acceleration.cpp:151-160 */
+        s->eval_drag = s->eval_mig1 = s->eval_mig2 = 0;
+        s->gas_dead = 1;
+    }
+    return 0;
+}
+
+int cr_gravity(cr_system *s, const double *y, double *f)
+{
+    if (s->barycentric) gravity_bc(s, y, f);
+    else gravity_ac(s, y, f);
+    return 0;
+}
+
+/* ---- integrator state ------------------------------------------------ */
+void cr_set_state(cr_system *s, const double *y0) { memcpy(s->y0, y0, (size_t)s->n*48); }
+void cr_set_accel(cr_system *s, const double *a) { memcpy(s->accel, a, (size_t)s->n*48); }
+
+void cr_get_array(const cr_system *s, int32_t which, double *out)
+{
+    const double *src = s->y0;
+    switch (which) {
+    case SOL_ARRAY_Y0: src = s->y0; break;
+    case SOL_ARRAY_Y: src = s->y; break;
+    case SOL_ARRAY_ACCEL: src = s->accel; break;
+    case SOL_ARRAY_YSCALE: src = s->yscale; break;
+    case SOL_ARRAY_ERROR: src = s->error; break;
+    }
+    memcpy(out, src, (size_t)s->n*48);
+}
+
+/* ---- Runge-Kutta-Fehlberg 7(8) --------------------------------------- */
+/* Fehlberg's coefficients (NASA TR R-381, 1972) as set in
+ * RungeKuttaFehlberg78::RungeKuttaFehlberg78, RungeKuttaFehlberg78.cpp:41-56;
+ * row s lists D_{s,l}. */
+static const double RKF_D[13][12] = {
+    {0},
+    {2.0/27.0},
+    {1.0/36.0, 1.0/12.0},
+    {1.0/24.0, 0, 1.0/8.0},
+    {5.0/12.0, 0, -25.0/16.0, 25.0/16.0},
+    {1.0/20.0, 0, 0, 1.0/4.0, 1.0/5.0},
+    {-25.0/108.0, 0, 0, 125.0/108.0, -65.0/27.0, 125.0/54.0},
+    {31.0/300.0, 0, 0, 0, 61.0/225.0, -2.0/9.0, 13.0/900.0},
+    {2.0, 0, 0, -53.0/6.0, 704.0/45.0, -107.0/9.0, 67.0/90.0, 3.0},
+    {-91.0/108.0, 0, 0, 23.0/108.0, -976.0/135.0, 311.0/54.0, -19.0/60.0, 17.0/6.0, -1.0/12.0},
+    {2383.0/4100.0, 0, 0, -341.0/164.0, 4496.0/1025.0, -301.0/82.0, 2133.0/4100.0, 45.0/82.0, 45.0/164.0, 18.0/41.0},
+    {3.0/205.0, 0, 0, 0, 0, -6.0/41.0, -3.0/205.0, -3.0/41.0, 3.0/41.0, 6.0/41.0},
+    {-1777.0/4100.0, 0, 0, -341.0/164.0, 4496.0/1025.0, -289.0/82.0, 2193.0/4100.0, 51.0/82.0, 33.0/164.0, 12.0/41.0, 0, 1.0},
+};
+
+/* RungeKuttaFehlberg78::Step, RungeKuttaFehlberg78.cpp:147-250.  The
+ * combination sums terms in ascending l, skipping the zero entries, which
+ * is the reference's left-to-right order. */
+static int rkf78_step(cr_system *s)
+{
+    const size_t nv = 6*(size_t)s->n;
+    double *fk[13];
+    double *yt = malloc(nv*8);
+    fk[0] = s->accel;
+    for (int k = 1; k < 13; k++) fk[k] = malloc(nv*8);
+    const double h = s->h, t = s->time;
+    for (int st = 1; st < 13; st++) {
+        for (size_t i = 0; i < nv; i++) {
+            double acc = 0.0; int first = 1;
+            for (int l = 0; l < st; l++) {
+                double d = RKF_D[st][l];
+                if (d == 0.0) continue;
+                if (first) { acc = d*fk[l][i]; first = 0; }
+                else acc += d*fk[l][i];
+            }
+            yt[i] = s->y0[i] + h*acc;
+        }
+        cr_compute(s, t, yt, fk[st]);                    /* all stages at t (H4) */
+    }
+    const double b0 = 41.0/840.0, b5 = 34.0/105.0, b6 = 9.0/35.0, b8 = 9.0/280.0, b10 = 41.0/840.0;
+    for (size_t i = 0; i < nv; i++)
+        s->y[i] = s->y0[i] + h*(b0*fk[0][i] + b5*fk[5][i] + b6*(fk[6][i] + fk[7][i]) +
+                                b8*(fk[8][i] + fk[9][i]) + b10*fk[10][i]);
+    for (size_t i = 0; i < nv; i++)
+        s->error[i] = h*fabs(fk[0][i] + fk[10][i] - fk[11][i] - fk[12][i])*41.0/840.0;
+    for (int k = 1; k < 13; k++) free(fk[k]);
+    free(yt);
+    return 0;
+}
+
+static double rkf78_error_max(const cr_system *s, double eps)
+{   /* RungeKuttaFehlberg78::GetErrorMax, :252-262 */
+    double m = 0.0;
+    const size_t nv = 6*(size_t)s->n;
+    for (size_t i = 0; i < nv; i++) {
+        double e = fabs(s->error[i] / s->yscale[i]);
+        if (e > m) m = e;
+    }
+    return m / eps;
+}
+
+static void swap_state(cr_system *s) { double *t = s->y0; s->y0 = s->y; s->y = t; }
+
+/* RungeKuttaFehlberg78::Driver, RungeKuttaFehlberg78.cpp:66-140 */
+static int rkf78_driver(cr_system *s, double eps, sol_timeline *tl)
+{
+    const size_t nv = 6*(size_t)s->n;
+    const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = 1.89e-4, TINY = 1.0e-30;
+    s->time = tl->time;
+    s->h = tl->hNext;
+    s->eval_drag = s->eval_mig1 = s->eval_mig2 = 1;
+    cr_compute(s, tl->time, s->y0, s->accel);
+    for (size_t i = 0; i < nv; i++)
+        s->yscale[i] = fabs(s->y0[i]) + fabs(s->h * s->accel[i]) + TINY;
+    s->eval_mig1 = s->eval_mig2 = 0;
+    double errmax = 0.0;
+    int result = 0;
+    for (;;) {
+        rkf78_step(s);
+        errmax = rkf78_error_max(s, eps);
+        if (errmax < 1.0) { tl->hDid = s->h; result = 0; break; }
+        double htemp = SAFETY * s->h * pow(errmax, PSHRNK);
+        s->h = fabs(htemp) > fabs(0.1*s->h) ? htemp : 0.1*s->h;
+        double tnew = tl->time + s->h;
+        if (tnew == tl->time) { result = SOL_ERR_UNDERFLOW; break; }
+    }
+    if (result == 0) {
+        tl->time += tl->hDid;
+        s->time = tl->time;
+        tl->hNext = errmax > ERRCON ? (SAFETY * s->h * pow(errmax, PGROW)) : (5.0 * s->h);
+        s->h = tl->hNext;
+        swap_state(s);
+    }
+    return result;
+}
+
+/* ---- classical Runge-Kutta 4 ----------------------------------------- */
+/* RungeKutta4::Step, RungeKutta4.cpp:58-130 */
+static int rk4_step(cr_system *s)
+{
+    const size_t nv = 6*(size_t)s->n;
+    double *f1 = malloc(nv*8), *f2 = malloc(nv*8), *f3 = malloc(nv*8), *yt = malloc(nv*8);
+    const double h = s->h, t = s->time;
+    const double *f0 = s->accel;
+    for (size_t i = 0; i < nv; i++) yt[i] = s->y0[i] + h*(0.5*f0[i]);
+    cr_compute(s, t + 0.5*h, yt, f1);
+    for (size_t i = 0; i < nv; i++) yt[i] = s->y0[i] + h*(0.5*f1[i]);
+    cr_compute(s, t + 0.5*h, yt, f2);
+    for (size_t i = 0; i < nv; i++) yt[i] = s->y0[i] + h*(1.0*f2[i]);
+    cr_compute(s, t + 1.0*h, yt, f3);
+    const double b1 = 1.0/6.0, b2 = 1.0/3.0, b3 = 1.0/3.0, b4 = 1.0/6.0;
+    for (size_t i = 0; i < nv; i++)
+        s->y[i] = s->y0[i] + h*(b1*f0[i] + b2*f1[i] + b3*f2[i] + b4*f3[i]);
+    free(f1); free(f2); free(f3); free(yt);
+    return 0;
+}
+
+/* RungeKutta4::Driver, RungeKutta4.cpp:20-56 */
+static int rk4_driver(cr_system *s, sol_timeline *tl)
+{
+    s->time = tl->time;
+    s->h = tl->hNext;
+    s->eval_drag = s->eval_mig1 = s->eval_mig2 = 1;
+    cr_compute(s, tl->time, s->y0, s->accel);
+    s->eval_mig1 = s->eval_mig2 = 0;
+    rk4_step(s);
+    tl->hDid = s->h;
+    tl->time += tl->hDid;
+    s->time = tl->time;
+    tl->hNext = s->h;
+    swap_state(s);
+    return 0;
+}
+
+/* ---- Dormand-Prince RKN 7(6) ----------------------------------------- */
+/* Coefficients of DormandPrince::DormandPrince, DormandPrince.cpp:26-124
+ * (Dormand & Prince, Celest. Mech. 18 (1978) 223-232). */
+typedef struct { double a[9][8], b[9], bdh[9], c[9]; } dp_tab;
+
+static void dp_coeffs(dp_tab *T)
+{
+    double sq = sqrt(21.0);
+    memset(T, 0, sizeof(*T));
+    T->b[0] = 1.0/20.0; T->b[4] = 8.0/45.0; T->b[5] = 7.0*(7.0 + sq)/360.0;
+    T->b[6] = 7.0*(7.0 - sq)/360.0; T->b[7] = -1.0/20.0; T->b[8] = 1.0/20.0;
+    T->bdh[0] = 1.0/20.0; T->bdh[4] = 16.0/45.0; T->bdh[5] = 49.0/180.0;
+    T->bdh[6] = 49.0/180.0; T->bdh[7] = 1.0/20.0;
+    T->c[1] = 1.0/10.0; T->c[2] = 1.0/5.0; T->c[3] = 3.0/8.0; T->c[4] = 1.0/2.0;
+    T->c[5] = (7.0 - sq)/14.0; T->c[6] = (7.0 + sq)/14.0; T->c[7] = 1.0; T->c[8] = 1.0;
+    T->a[1][0] = 1.0/200.0;
+    T->a[2][0] = 1.0/150.0; T->a[2][1] = 1.0/75.0;
+    T->a[3][0] = 171.0/8192.0; T->a[3][1] = 45.0/4096.0; T->a[3][2] = 315.0/8192.0;
+    T->a[4][0] = 5.0/288.0; T->a[4][1] = 25.0/528.0; T->a[4][2] = 25.0/672.0; T->a[4][3] = 16.0/693.0;
+    T->a[5][0] = (1003.0 - 205.0*sq)/12348.0; T->a[5][1] = -25.0*(751.0 - 173.0*sq)/90552.0;
+    T->a[5][2] = 25.0*(624.0 - 137.0*sq)/43218.0; T->a[5][3] = -128.0*(361.0 - 79.0*sq)/237699.0;
+    T->a[5][4] = (3411.0 - 745.0*sq)/24696.0;
+    T->a[6][0] = (793.0 + 187.0*sq)/12348.0; T->a[6][1] = -25.0*(331.0 + 113.0*sq)/90552.0;
+    T->a[6][2] = 25.0*(1044.0 + 247.0*sq)/43218.0; T->a[6][3] = -128.0*(14885.0 + 3779.0*sq)/9745659.0;
+    T->a[6][4] = (3327.0 + 797.0*sq)/24696.0; T->a[6][5] = -(581.0 + 127.0*sq)/1722.0;
+    T->a[7][0] = -(157.0 - 3.0*sq)/378.0; T->a[7][1] = 25.0*(143.0 - 10.0*sq)/2772.0;
+    T->a[7][2] = -25.0*(876.0 + 55.0*sq)/3969.0; T->a[7][3] = 1280.0*(913.0 + 18.0*sq)/596673.0;
+    T->a[7][4] = -(1353.0 + 26.0*sq)/2268.0; T->a[7][5] = 7.0*(1777.0 + 377.0*sq)/4428.0;
+    T->a[7][6] = 7.0*(5.0 - sq)/36.0;
+    T->a[8][0] = 1.0/20.0; T->a[8][4] = 8.0/45.0; T->a[8][5] = 7.0*(7.0 + sq)/360.0;
+    T->a[8][6] = 7.0*(7.0 - sq)/360.0;
+}
+
+/* DormandPrince::Step2, DormandPrince.cpp:246-491.  Stage k's acceleration
+ * sum runs over l < k (stage 8 over l = 0, 4, 5, 6), left to right. */
+static int dp_step(cr_system *s)
+{
+    dp_tab T; dp_coeffs(&T);
+    const int n = s->n;
+    const size_t nv = 6*(size_t)n;
+    double *f[9];
+    double *yt = calloc(nv, 8);
+    f[0] = s->accel;
+    for (int k = 1; k < 9; k++) f[k] = calloc(nv, 8);
+    const double h = s->h, h2 = h*h;
+    for (int k = 1; k < 9; k++) {
+        double tt = s->time + T.c[k]*h;
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < 3; j++) {
+                int q = 6*i + j;
+                double var = T.a[k][0]*f[0][q+3];
+                int lmax = (k == 8) ? 7 : k;
+                for (int l = 1; l < lmax; l++) {
+                    if (k == 8 && (l < 4)) continue;
+                    var += T.a[k][l]*f[l][q+3];
+                }
+                yt[q] = s->y0[q] + T.c[k]*h*s->y0[q+3] + h2*(var);
+                yt[q+3] = s->y0[q+3] + h*(var);
+            }
+        }
+        cr_compute(s, tt, yt, f[k]);
+    }
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < 3; j++) {
+            int q = 6*i + j;
+            s->y[q] = s->y0[q] + h*s->y0[q+3] + h2*(T.b[0]*f[0][q+3] + T.b[4]*f[4][q+3] + T.b[5]*f[5][q+3] +
+                                                     T.b[6]*f[6][q+3] + T.b[7]*f[7][q+3] + T.b[8]*f[8][q+3]);
+            s->error[q] = h2 * fabs(f[7][q+3] - f[8][q+3]) / 20.0;
+            s->y[q+3] = s->y0[q+3] + h*(T.bdh[0]*f[0][q+3] + T.bdh[4]*f[4][q+3] + T.bdh[5]*f[5][q+3] +
+                                        T.bdh[6]*f[6][q+3] + T.bdh[7]*f[7][q+3]);
+            s->error[q+3] = 0.0;
+        }
+    }
+    for (int k = 1; k < 9; k++) free(f[k]);
+    free(yt);
+    return 0;
+}
+
+static double dp_error_max(const cr_system *s)
+{   /* DormandPrince::GetErrorMax, DormandPrince.cpp:493-503 */
+    double m = 0.0;
+    const size_t nv = 6*(size_t)s->n;
+    for (size_t i = 0; i < nv; i++) {
+        double e = fabs(s->error[i]);
+        if (e > m) m = e;
+    }
+    return m;
+}
+
+/* DormandPrince::Driver, DormandPrince.cpp:126-170 */
+static int dp_driver(cr_system *s, double eps, sol_timeline *tl)
+{
+    const int maxIter = 10;
+    s->time = tl->time;
+    s->eval_drag = s->eval_mig1 = s->eval_mig2 = 1;
+    cr_compute(s, tl->time, s->y0, s->accel);
+    s->eval_mig1 = s->eval_mig2 = 0;
+    int iter = 0;
+    double errmax = 0.0;
+    do {
+        iter++;
+        s->h = tl->hNext;
+        dp_step(s);
+        errmax = dp_error_max(s);
+        tl->hDid = s->h;
+        tl->hNext = errmax < 1.0e-20 ? 2.0*s->h : 0.9*s->h*pow(eps / errmax, 1.0/7.0);
+    } while (errmax > eps && iter <= maxIter);
+    if (iter > maxIter) return SOL_ERR_MAXITER;
+    tl->time += tl->hDid;
+    s->time = tl->time;
+    swap_state(s);
+    return 0;
+}
+
+int cr_driver(cr_system *s, int32_t integrator, double eps, sol_timeline *tl, double *body_h)
+{
+    int r;
+    switch (integrator) {
+    case SOL_RUNGE_KUTTA_FEHLBERG78: r = rkf78_driver(s, eps, tl); break;
+    case SOL_RUNGE_KUTTA4: r = rk4_driver(s, tl); break;
+    case SOL_DORMAND_PRINCE: r = dp_driver(s, eps, tl); break;
+    default: return SOL_ERR_UNSUPPORTED;
+    }
+    if (body_h) *body_h = s->h;
+    return r;
+}
+
+int cr_step(cr_system *s, int32_t integrator, double time, double h)
+{
+    s->time = time; s->h = h;
+    switch (integrator) {
+    case SOL_RUNGE_KUTTA_FEHLBERG78: return rkf78_step(s);
+    case SOL_RUNGE_KUTTA4: return rk4_step(s);
+    case SOL_DORMAND_PRINCE: return dp_step(s);
+    default: return SOL_ERR_UNSUPPORTED;
+    }
+}
+
+double cr_error_max(const cr_system *s, int32_t integrator, double eps)
+{
+    if (integrator == SOL_DORMAND_PRINCE) return dp_error_max(s);
+    return rkf78_error_max(s, eps);
+}
+
+/* Simulator::Integrate + DecisionMaking without events and file output,
+ * Solaris/Simulator.cpp:121-178, :181-248. */
+int cr_integrate(cr_system *s, int32_t integrator, double eps, sol_timeline *tl,
+                 int64_t max_steps, sol_run_stats *st)
+{
+    const size_t nv = 6*(size_t)s->n;
+    int64_t steps = 0;
+    double ev0 = s->evals;
+    double ipe = cr_interactions_per_evaluation(s);
+    int r = 0;
+    for (;;) {
+        double hb;
+        r = cr_driver(s, integrator, eps, tl, &hb);
+        if (r) break;
+        steps++;
+        /* DecisionMaking, :183-245 */
+        tl->elapsedTime += tl->hDid;
+        tl->lastSave += tl->hDid;
+        tl->lastNSteps += tl->hDid;
+        double actual = 1000.0*C_YEAR_TO_DAY*tl->millenium + tl->time;
+        int stop = 0;
+        if (fabs(actual) >= fabs(tl->length)) stop = 1;
+        else if (s->n <= 1) stop = 1;
+        else {
+            if (fabs(actual + tl->hNext) > fabs(tl->length)) tl->hNext = tl->length - actual;
+            if (fabs(tl->lastSave) >= fabs(tl->output)) tl->lastSave = 0.0;
+            if (fabs(tl->lastSave + tl->hNext) > fabs(tl->output)) tl->hNext = tl->output - tl->lastSave;
+        }
+        if (steps % C_CHECK_FOR_SM == 0) {                      /* :159-162 */
+            for (size_t i = 0; i < nv; i++) {
+                if (fabs(s->y[i]) < C_SMALLEST_NUMBER) s->y[i] = 0.0;
+                if (fabs(s->y0[i]) < C_SMALLEST_NUMBER) s->y0[i] = 0.0;
+            }
+        }
+        if (stop) break;
+        if (max_steps > 0 && steps >= max_steps) break;
+    }
+    if (st) {
+        st->accepted_steps = steps;
+        st->force_evaluations = (int64_t)(s->evals - ev0);
+        st->interactions = (s->evals - ev0) * ipe;
+        st->attempts = 0;
+        st->device_ms = 0.0;
+    }
+    return r;
+}
+
+/* Calculate::Integrals, Solaris/Calculate.cpp:43-172 on y0. */
+void cr_integrals(const cr_system *s, double *out)
+{
+    const int n = s->n, nm = n_massive(&s->nb);
+    const double *y = s->y0;
+    double M = 0.0;
+    for (int i = 0; i < nm; i++) M += s->mass[i];               /* TotalMass */
+    out[0] = M;
+    double bc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; i++)                                  /* PhaseOfBC */
+        for (int j = 0; j < 6; j++) bc[j] += s->mass[i] * y[6*i + j];
+    for (int j = 0; j < 6; j++) out[1 + j] = bc[j] / M;
+    out[7] = sqrt(SQR(out[1]) + SQR(out[2]) + SQR(out[3]));
+    out[8] = sqrt(SQR(out[4]) + SQR(out[5]) + SQR(out[6]));
+    double cx = 0, cy = 0, cz = 0;                               /* AngularMomentum */
+    for (int i = 0; i < n; i++) {
+        const double *p = y + 6*i;
+        double m = s->mass[i];
+        cx += m*(p[1]*p[5] - p[2]*p[4]);
+        cy += m*(p[2]*p[3] - p[0]*p[5]);
+        cz += m*(p[0]*p[4] - p[1]*p[3]);
+    }
+    out[9] = cx; out[10] = cy; out[11] = cz;
+    out[12] = sqrt(cx*cx + cy*cy + cz*cz);
+    double ke = 0.0;                                             /* KineticEnergy */
+    for (int i = 0; i < n; i++) {
+        const double *p = y + 6*i;
+        double v2 = SQR(p[3]) + SQR(p[4]) + SQR(p[5]);
+        ke += 0.5*s->mass[i] * v2;
+    }
+    double pe = 0.0;                                             /* PotentialEnergy */
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) {
+            if (i == j) continue;
+            double dx = y[6*j] - y[6*i], dy = y[6*j+1] - y[6*i+1], dz = y[6*j+2] - y[6*i+2];
+            double rij = sqrt(dx*dx + dy*dy + dz*dz);
+            pe += s->mass[i]*s->mass[j]/rij;
+        }
+    }
+    pe *= 0.5*C_GAUSS2;
+    out[13] = ke; out[14] = pe; out[15] = ke - pe;
+}

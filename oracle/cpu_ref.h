@@ -1,0 +1,72 @@
+/*
+ * cpu_ref.h -- the oracle: a plain-C FP64 restatement of the Solaris
+ * force-evaluation and integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only as the checker or the
+ * timed CPU baseline.  The product library never links or calls it.
+ *
+ * The API mirrors include/solaris_amd.h so that a test can drive the GPU
+ * library and this restatement with the same calls.  Every function cites
+ * the reference file:line it follows.
+ */
+#ifndef SOLARIS_CPU_REF_H_
+#define SOLARIS_CPU_REF_H_
+
+#include <stdint.h>
+#include "../include/solaris_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cr_system cr_system;
+
+cr_system *cr_create(const sol_nbodies *nb, int32_t barycentric);
+void       cr_destroy(cr_system *s);
+void       cr_set_bodies(cr_system *s, const sol_body_params *p);
+void       cr_set_nebula(cr_system *s, const sol_gas_component *gas);
+void       cr_set_evaluate(cr_system *s, int32_t drag, int32_t mig1, int32_t mig2);
+void       cr_get_evaluate(const cr_system *s, int32_t *drag, int32_t *mig1, int32_t *mig2);
+void       cr_set_softening(cr_system *s, double softening);
+void       cr_set_source_ranges(cr_system *s, int32_t end_massive, int32_t end_other);
+void       cr_set_mig_type(cr_system *s, const int32_t *mig_type);
+
+int        cr_compute(cr_system *s, double t, const double *y, double *dydt);
+/* Gravity only (GravityAC / GravityBC), no gas terms. */
+int        cr_gravity(cr_system *s, const double *y, double *dydt);
+void       cr_get_side_outputs(const cr_system *s, double *rm3, int32_t *index_nn,
+                               double *distance_nn, int32_t *mig_type);
+
+void       cr_set_state(cr_system *s, const double *y0);
+void       cr_set_accel(cr_system *s, const double *accel);
+void       cr_get_array(const cr_system *s, int32_t which, double *out);
+int        cr_driver(cr_system *s, int32_t integrator, double epsilon,
+                     sol_timeline *tl, double *body_h);
+int        cr_step(cr_system *s, int32_t integrator, double time, double h);
+double     cr_error_max(const cr_system *s, int32_t integrator, double epsilon);
+int        cr_integrate(cr_system *s, int32_t integrator, double epsilon,
+                        sol_timeline *tl, int64_t max_steps, sol_run_stats *stats);
+void       cr_integrals(const cr_system *s, double *out16);
+double     cr_interactions_per_evaluation(const cr_system *s);
+
+/* Single gas/orbit helpers, exposed for the reference's own unit-test
+ * vectors (src/Solaris.NBody.Cuda.Test/unit_test.cpp:558-699,
+ * Test/Test.cpp:421-532). */
+void   cr_circular_velocity(double mu, const double *r3, double *v3);
+void   cr_gas_velocity(const sol_gas_component *g, double mu, const double *r3, double *v3);
+double cr_gas_density_at(const sol_gas_component *g, const double *r3);
+double cr_temperature_cmu(const sol_gas_component *g, double mc, double r);
+double cr_mean_thermal_speed_cmu(const sol_gas_component *g, double mc, double r);
+int    cr_orbital_a_e(double mu, const double *rv6, double *a, double *e);
+/* GasComponent() defaults (Solaris/GasComponent.cpp:9-34). */
+void   cr_default_gas(sol_gas_component *g);
+/* Solaris/Constants.h values by name ("Gauss2", "GramPerCm3ToSolarPerAu3"...);
+ * NaN for an unknown name. */
+double cr_constant(const char *name);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,222 @@
+"""ctypes access to the two CPU checkers.  TEST INFRASTRUCTURE ONLY.
+
+``Oracle(scn)`` drives oracle/libsolcpu.so (the C restatement, cpu_ref.c);
+``Oracle(scn, lib="omp")`` the OpenMP build of the same source; and
+``Reference(scn)`` drives oracle/_ref/libsolref.so, the reference's own
+Solaris/*.cpp compiled by oracle/Makefile (only where that was built).
+Both expose the same methods as solaris_amd.Context so that tests can run
+one call sequence against all three.  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from solaris_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBS = {
+    "cpu": os.path.join(HERE, "libsolcpu.so"),
+    "omp": os.path.join(HERE, "libsolcpu_omp.so"),
+    "ref": os.path.join(HERE, "_ref", "libsolref.so"),
+}
+_loaded = {}
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int32)
+
+
+def _load(kind: str):
+    if kind in _loaded:
+        return _loaded[kind]
+    path = LIBS[kind]
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (make -C oracle {'ref' if kind == 'ref' else 'oracle'})")
+    lib = C.CDLL(path)
+    p = "rf_" if kind == "ref" else "cr_"
+    V = C.c_void_p
+    sig = {
+        "create": (V, [C.POINTER(_abi.NBodies), C.c_int32]),
+        "destroy": (None, [V]),
+        "set_bodies": (None, [V, C.POINTER(_abi.BodyParams)]),
+        "set_nebula": (None, [V, C.POINTER(_abi.GasComponent)]),
+        "set_evaluate": (None, [V, C.c_int32, C.c_int32, C.c_int32]),
+        "set_mig_type": (None, [V, I]),
+        "compute": (C.c_int, [V, C.c_double, D, D]),
+        "get_side_outputs": (None, [V, D, I, D, I]),
+        "set_state": (None, [V, D]),
+        "set_accel": (None, [V, D]),
+        "get_array": (None, [V, C.c_int32, D]),
+        "driver": (C.c_int, [V, C.c_int32, C.c_double, C.POINTER(_abi.TimeLine), D]),
+        "step": (C.c_int, [V, C.c_int32, C.c_double, C.c_double]),
+        "error_max": (C.c_double, [V, C.c_int32, C.c_double]),
+        "integrals": (None, [V, D]),
+    }
+    if kind != "ref":
+        sig.update({
+            "set_softening": (None, [V, C.c_double]),
+            "set_source_ranges": (None, [V, C.c_int32, C.c_int32]),
+            "gravity": (C.c_int, [V, D, D]),
+            "integrate": (C.c_int, [V, C.c_int32, C.c_double, C.POINTER(_abi.TimeLine),
+                                    C.c_int64, C.POINTER(_abi.RunStats)]),
+            "interactions_per_evaluation": (C.c_double, [V]),
+            "circular_velocity": (None, [C.c_double, D, D]),
+            "gas_velocity": (None, [C.POINTER(_abi.GasComponent), C.c_double, D, D]),
+            "gas_density_at": (C.c_double, [C.POINTER(_abi.GasComponent), D]),
+            "temperature_cmu": (C.c_double, [C.POINTER(_abi.GasComponent), C.c_double, C.c_double]),
+            "mean_thermal_speed_cmu": (C.c_double, [C.POINTER(_abi.GasComponent), C.c_double, C.c_double]),
+            "orbital_a_e": (C.c_int, [C.c_double, D, D, D]),
+            "default_gas": (None, [C.POINTER(_abi.GasComponent)]),
+            "constant": (C.c_double, [C.c_char_p]),
+        })
+    else:
+        sig.update({
+            "default_gas": (None, [C.POINTER(_abi.GasComponent)]),
+            "gas_velocity": (None, [C.POINTER(_abi.GasComponent), C.c_double, D, D]),
+            "gas_density_at": (C.c_double, [C.POINTER(_abi.GasComponent), D]),
+        })
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, p + name)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    _loaded[kind] = fns
+    return fns
+
+
+def available(kind: str) -> bool:
+    return os.path.exists(LIBS[kind])
+
+
+def _dp(a):
+    return a.ctypes.data_as(D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(I)
+
+
+class _Base:
+    """Same method names as solaris_amd.Context."""
+
+    def __init__(self, scn, kind, barycentric=False):
+        self.f = _load(kind)
+        self.kind = kind
+        self.n = scn.n
+        self._nb = _abi.nbodies(scn.counts)
+        self.h = self.f["create"](C.byref(self._nb), int(barycentric))
+        if not self.h:
+            raise RuntimeError("create failed")
+        self.set_bodies(scn)
+        self.set_nebula(scn.gas)
+
+    def close(self):
+        if self.h:
+            self.f["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bodies(self, scn):
+        self._keep, self._bp = _abi.body_params(scn)
+        self.f["set_bodies"](self.h, C.byref(self._bp))
+
+    def set_nebula(self, gas):
+        self._gas = _abi.gas_component(gas) if gas is not None else None
+        self.f["set_nebula"](self.h, C.byref(self._gas) if self._gas is not None else None)
+
+    def set_evaluate(self, drag, m1, m2):
+        self.f["set_evaluate"](self.h, int(drag), int(m1), int(m2))
+
+    def set_mig_type(self, mig):
+        m = np.ascontiguousarray(mig, dtype=np.int32)
+        self.f["set_mig_type"](self.h, _ip(m))
+
+    def compute(self, t, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(y)
+        r = self.f["compute"](self.h, float(t), _dp(y), _dp(out))
+        if r:
+            raise RuntimeError(f"compute returned {r}")
+        return out.reshape(-1, 6)
+
+    def side_outputs(self):
+        rm3 = np.zeros(self.n); nn = np.zeros(self.n, np.int32)
+        dist = np.zeros(self.n); mig = np.zeros(self.n, np.int32)
+        self.f["get_side_outputs"](self.h, _dp(rm3), _ip(nn), _dp(dist), _ip(mig))
+        return rm3, nn, dist, mig
+
+    def set_state(self, y0):
+        y = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1)
+        self.f["set_state"](self.h, _dp(y))
+
+    def set_accel(self, a):
+        y = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        self.f["set_accel"](self.h, _dp(y))
+
+    def get_array(self, which):
+        out = np.zeros(6 * self.n)
+        self.f["get_array"](self.h, int(which), _dp(out))
+        return out.reshape(-1, 6)
+
+    def driver(self, integrator, eps, tl):
+        bh = C.c_double(0.0)
+        r = self.f["driver"](self.h, int(integrator), float(eps), C.byref(tl), C.byref(bh))
+        return r, bh.value
+
+    def step(self, integrator, time, h):
+        return self.f["step"](self.h, int(integrator), float(time), float(h))
+
+    def error_max(self, integrator, eps):
+        return self.f["error_max"](self.h, int(integrator), float(eps))
+
+    def integrals(self):
+        out = np.zeros(16)
+        self.f["integrals"](self.h, _dp(out))
+        return out
+
+
+class Oracle(_Base):
+    def __init__(self, scn, lib="cpu", barycentric=False):
+        super().__init__(scn, lib, barycentric)
+        if scn.softening:
+            self.set_softening(scn.softening)
+
+    def set_softening(self, s):
+        self.f["set_softening"](self.h, float(s))
+
+    def set_source_ranges(self, a, b):
+        self.f["set_source_ranges"](self.h, int(a), int(b))
+
+    def gravity(self, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(y)
+        self.f["gravity"](self.h, _dp(y), _dp(out))
+        return out.reshape(-1, 6)
+
+    def integrate(self, integrator, eps, tl, max_steps=0):
+        st = _abi.RunStats()
+        r = self.f["integrate"](self.h, int(integrator), float(eps), C.byref(tl),
+                                int(max_steps), C.byref(st))
+        return r, st
+
+    def interactions_per_evaluation(self):
+        return self.f["interactions_per_evaluation"](self.h)
+
+
+class Reference(_Base):
+    def __init__(self, scn, barycentric=False):
+        super().__init__(scn, "ref", barycentric)
+
+
+def helper(kind="cpu"):
+    """Direct access to the single-function entry points."""
+    return _load(kind)

@@ -1,0 +1,99 @@
+"""ctypes mirrors of the plain-data structs in include/solaris_amd.h."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int32)
+
+
+class NBodies(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "central_body", "giant_planet", "rocky_planet", "proto_planet",
+        "super_planetesimal", "planetesimal", "test_particle", "total")]
+
+
+class PowerLaw(C.Structure):
+    _fields_ = [("c", C.c_double), ("index", C.c_double)]
+
+
+class GasComponent(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("mean_molecular_weight", C.c_double),
+                ("particle_diameter", C.c_double), ("type", C.c_int32),
+                ("reserved", C.c_int32), ("time_scale", C.c_double), ("t0", C.c_double),
+                ("t1", C.c_double), ("inner_edge", C.c_double), ("a", C.c_double),
+                ("eta", PowerLaw), ("tau", PowerLaw), ("scale_height", PowerLaw),
+                ("density", PowerLaw), ("mean_free_path", PowerLaw)]
+
+
+class BodyParams(C.Structure):
+    _fields_ = [("mass", D), ("radius", D), ("density", D), ("cd", D),
+                ("gamma_stokes", D), ("gamma_epstein", D), ("mig_stop_at", D),
+                ("type", I), ("mig_type", I)]
+
+
+class TimeLine(C.Structure):
+    _fields_ = [("start", C.c_double), ("length", C.c_double), ("output", C.c_double),
+                ("save", C.c_double), ("time", C.c_double), ("hNext", C.c_double),
+                ("hDid", C.c_double), ("elapsedTime", C.c_double), ("lastSave", C.c_double),
+                ("lastNSteps", C.c_double), ("millenium", C.c_int32), ("reserved", C.c_int32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("accepted_steps", C.c_int64), ("attempts", C.c_int64),
+                ("force_evaluations", C.c_int64), ("interactions", C.c_double),
+                ("device_ms", C.c_double)]
+
+
+def nbodies(counts: dict) -> NBodies:
+    nb = NBodies()
+    for k, _ in NBodies._fields_:
+        setattr(nb, k, int(counts.get(k, 0)))
+    return nb
+
+
+def body_params(scn):
+    """Returns (keep-alive arrays, BodyParams) for a Scenario."""
+    keep = {}
+    bp = BodyParams()
+    for name in ("mass", "radius", "density", "cd", "gamma_stokes", "gamma_epstein", "mig_stop_at"):
+        a = np.ascontiguousarray(getattr(scn, name), dtype=np.float64)
+        keep[name] = a
+        setattr(bp, name, a.ctypes.data_as(D))
+    for name in ("type", "mig_type"):
+        a = np.ascontiguousarray(getattr(scn, name), dtype=np.int32)
+        keep[name] = a
+        setattr(bp, name, a.ctypes.data_as(I))
+    return keep, bp
+
+
+def gas_component(g: dict) -> GasComponent:
+    s = GasComponent()
+    for k in ("alpha", "mean_molecular_weight", "particle_diameter", "time_scale",
+              "t0", "t1", "inner_edge", "a"):
+        setattr(s, k, float(g.get(k, 0.0)))
+    s.type = int(g.get("type", 0))
+    for k in ("eta", "tau", "scale_height", "density", "mean_free_path"):
+        c, idx = g[k]
+        setattr(s, k, PowerLaw(float(c), float(idx)))
+    return s
+
+
+def gas_to_dict(s: GasComponent) -> dict:
+    d = {k: getattr(s, k) for k in ("alpha", "mean_molecular_weight", "particle_diameter",
+                                    "time_scale", "t0", "t1", "inner_edge", "a", "type")}
+    for k in ("eta", "tau", "scale_height", "density", "mean_free_path"):
+        p = getattr(s, k)
+        d[k] = (p.c, p.index)
+    return d
+
+
+def timeline(time=0.0, h_next=0.0, length=0.0, output=None) -> TimeLine:
+    tl = TimeLine()
+    tl.time = time
+    tl.hNext = h_next
+    tl.length = length
+    tl.output = length if output is None else output
+    return tl
