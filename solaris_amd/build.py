@@ -1,0 +1,83 @@
+"""Builds the in-tree libraries.
+
+  solaris_amd/libsolaris_amd.so   HIP kernels + the C ABI (hipcc, gfx950)
+  solaris_amd/libsolaris_dropin.so  C++ classes with the reference's
+                                    Acceleration / integrator / BodyData
+                                    interfaces on top of the C ABI (g++)
+
+Every HIP translation unit is compiled with packed FP32 instructions
+disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+DROPIN = os.path.join(HERE, "dropin")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(HERE, "libsolaris_amd.so")
+DROPIN_LIB = os.path.join(HERE, "libsolaris_dropin.so")
+
+ARCH = os.environ.get("SOLARIS_AMD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+HIP_SOURCES = ["pairs.hip", "stages.hip", "context.cpp"]
+
+
+def _run(cmd, quiet=True):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines()
+                    if "packed-fp32-ops' is not a recognized feature" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"build step failed ({p.returncode}): {' '.join(cmd)}\n{out}")
+    if out.strip() and not quiet:
+        print(out)
+    return out
+
+
+def build_hip(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
+        list(ex.map(lambda c: _run(c, not verbose), cmds))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], not verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_dropin(verbose: bool = False) -> str:
+    srcs = sorted(os.path.join(DROPIN, f) for f in os.listdir(DROPIN) if f.endswith(".cpp")) \
+        if os.path.isdir(DROPIN) else []
+    if not srcs:
+        return ""
+    tmp = DROPIN_LIB + ".tmp"
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{DROPIN}",
+          f"-I{os.path.join(ROOT, 'include')}", "-o", tmp, *srcs,
+          f"-L{HERE}", "-lsolaris_amd", "-Wl,-rpath,$ORIGIN"], not verbose)
+    os.replace(tmp, DROPIN_LIB)
+    return DROPIN_LIB
+
+
+def build_all(verbose: bool = False):
+    build_hip(verbose)
+    build_dropin(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-v" in sys.argv)
+    print("built", LIB)

@@ -1,0 +1,883 @@
+// context.cpp -- the C ABI of include/solaris_amd.h on one MI355X.
+//
+// A context owns the device mirror of BodyData (structure-of-arrays), the
+// stage buffers of the integrators, the pair-sum work plan and one HIP
+// stream.  The drivers follow the reference's control flow step for step
+// (RungeKuttaFehlberg78::Driver, RungeKutta4::Driver, DormandPrince::Driver)
+// with all array work on the device; only the scalar step-size decisions
+// run on the host, from one 8-byte error maximum per attempt.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sol_internal.h"
+
+using namespace sol;
+
+namespace {
+
+// Fehlberg 7(8) coefficients D_{s,l} (NASA TR R-381; as set in
+// RungeKuttaFehlberg78.cpp:41-56), RK4 tableau (RungeKutta4.cpp:60-70) and
+// Dormand-Prince RKN 7(6) (DormandPrince.cpp:37-123), packed at the offsets
+// stages.hip reads.
+void fill_coefficients(std::vector<double> &c)
+{
+    c.assign(275, 0.0);
+    const double D[13][12] = {
+        {0},
+        {2.0 / 27.0},
+        {1.0 / 36.0, 1.0 / 12.0},
+        {1.0 / 24.0, 0, 1.0 / 8.0},
+        {5.0 / 12.0, 0, -25.0 / 16.0, 25.0 / 16.0},
+        {1.0 / 20.0, 0, 0, 1.0 / 4.0, 1.0 / 5.0},
+        {-25.0 / 108.0, 0, 0, 125.0 / 108.0, -65.0 / 27.0, 125.0 / 54.0},
+        {31.0 / 300.0, 0, 0, 0, 61.0 / 225.0, -2.0 / 9.0, 13.0 / 900.0},
+        {2.0, 0, 0, -53.0 / 6.0, 704.0 / 45.0, -107.0 / 9.0, 67.0 / 90.0, 3.0},
+        {-91.0 / 108.0, 0, 0, 23.0 / 108.0, -976.0 / 135.0, 311.0 / 54.0, -19.0 / 60.0, 17.0 / 6.0, -1.0 / 12.0},
+        {2383.0 / 4100.0, 0, 0, -341.0 / 164.0, 4496.0 / 1025.0, -301.0 / 82.0, 2133.0 / 4100.0, 45.0 / 82.0,
+         45.0 / 164.0, 18.0 / 41.0},
+        {3.0 / 205.0, 0, 0, 0, 0, -6.0 / 41.0, -3.0 / 205.0, -3.0 / 41.0, 3.0 / 41.0, 6.0 / 41.0},
+        {-1777.0 / 4100.0, 0, 0, -341.0 / 164.0, 4496.0 / 1025.0, -289.0 / 82.0, 2193.0 / 4100.0, 51.0 / 82.0,
+         33.0 / 164.0, 12.0 / 41.0, 0, 1.0},
+    };
+    for (int s = 0; s < 13; ++s)
+        for (int l = 0; l < 12; ++l) c[0 + s * 12 + l] = D[s][l];
+    // RK4: a21 = a32 = 1/2, a43 = 1; b = (1/6, 1/3, 1/3, 1/6); c = (0, 1/2, 1/2, 1)
+    c[156 + 1 * 3 + 0] = 0.5;
+    c[156 + 2 * 3 + 1] = 0.5;
+    c[156 + 3 * 3 + 2] = 1.0;
+    c[168 + 0] = 1.0 / 6.0; c[168 + 1] = 1.0 / 3.0; c[168 + 2] = 1.0 / 3.0; c[168 + 3] = 1.0 / 6.0;
+    c[172 + 0] = 0.0; c[172 + 1] = 0.5; c[172 + 2] = 0.5; c[172 + 3] = 1.0;
+    // Dormand-Prince RKN 7(6)
+    const double sq = std::sqrt(21.0);
+    double a[9][8] = {};
+    a[1][0] = 1.0 / 200.0;
+    a[2][0] = 1.0 / 150.0; a[2][1] = 1.0 / 75.0;
+    a[3][0] = 171.0 / 8192.0; a[3][1] = 45.0 / 4096.0; a[3][2] = 315.0 / 8192.0;
+    a[4][0] = 5.0 / 288.0; a[4][1] = 25.0 / 528.0; a[4][2] = 25.0 / 672.0; a[4][3] = 16.0 / 693.0;
+    a[5][0] = (1003.0 - 205.0 * sq) / 12348.0; a[5][1] = -25.0 * (751.0 - 173.0 * sq) / 90552.0;
+    a[5][2] = 25.0 * (624.0 - 137.0 * sq) / 43218.0; a[5][3] = -128.0 * (361.0 - 79.0 * sq) / 237699.0;
+    a[5][4] = (3411.0 - 745.0 * sq) / 24696.0;
+    a[6][0] = (793.0 + 187.0 * sq) / 12348.0; a[6][1] = -25.0 * (331.0 + 113.0 * sq) / 90552.0;
+    a[6][2] = 25.0 * (1044.0 + 247.0 * sq) / 43218.0; a[6][3] = -128.0 * (14885.0 + 3779.0 * sq) / 9745659.0;
+    a[6][4] = (3327.0 + 797.0 * sq) / 24696.0; a[6][5] = -(581.0 + 127.0 * sq) / 1722.0;
+    a[7][0] = -(157.0 - 3.0 * sq) / 378.0; a[7][1] = 25.0 * (143.0 - 10.0 * sq) / 2772.0;
+    a[7][2] = -25.0 * (876.0 + 55.0 * sq) / 3969.0; a[7][3] = 1280.0 * (913.0 + 18.0 * sq) / 596673.0;
+    a[7][4] = -(1353.0 + 26.0 * sq) / 2268.0; a[7][5] = 7.0 * (1777.0 + 377.0 * sq) / 4428.0;
+    a[7][6] = 7.0 * (5.0 - sq) / 36.0;
+    a[8][0] = 1.0 / 20.0; a[8][4] = 8.0 / 45.0; a[8][5] = 7.0 * (7.0 + sq) / 360.0; a[8][6] = 7.0 * (7.0 - sq) / 360.0;
+    for (int k = 0; k < 9; ++k)
+        for (int l = 0; l < 8; ++l) c[176 + k * 8 + l] = a[k][l];
+    const double b[9] = {1.0 / 20.0, 0, 0, 0, 8.0 / 45.0, 7.0 * (7.0 + sq) / 360.0, 7.0 * (7.0 - sq) / 360.0,
+                         -1.0 / 20.0, 1.0 / 20.0};
+    const double bdh[9] = {1.0 / 20.0, 0, 0, 0, 16.0 / 45.0, 49.0 / 180.0, 49.0 / 180.0, 1.0 / 20.0, 0.0};
+    const double cc[9] = {0.0, 1.0 / 10.0, 1.0 / 5.0, 3.0 / 8.0, 1.0 / 2.0, (7.0 - sq) / 14.0, (7.0 + sq) / 14.0,
+                          1.0, 1.0};
+    for (int k = 0; k < 9; ++k) { c[248 + k] = b[k]; c[257 + k] = bdh[k]; c[266 + k] = cc[k]; }
+}
+
+double host_reduction_factor(const sol_gas_component &g, double t)
+{   // GasComponent::ReductionFactor, Solaris/GasComponent.cpp:36-61
+    switch (g.type) {
+    case SOL_GAS_CONSTANT: return 1.0;
+    case SOL_GAS_LINEAR:
+        if (t <= g.t0) return 1.0;
+        if (t > g.t0 && t <= g.t1) return 1.0 - (t - g.t0) / (g.t1 - g.t0);
+        return 0.0;
+    case SOL_GAS_EXPONENTIAL: return std::exp(-t / g.time_scale);
+    default: return 1.0;
+    }
+}
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct sol_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sol_nbodies nb{};
+    int n = 0, np = 0, nblk = 0, n_massive = 0;
+    int barycentric = 0;
+    int precision = SOL_PRECISION_FP64;
+    double softening = 0.0;
+    int end_massive = -1, end_other = -1;
+    bool has_gas = false, gas_dead = false;
+    sol_gas_component gas{};
+    GasDev gasdev{};
+    int eval_drag = 1, eval_mig1 = 1, eval_mig2 = 1;
+    int cu_count = 256;
+    int waves_per_simd = 8;
+    int chunk = 256;
+    PairPlan plan{};
+    double interactions_per_eval = 0.0;
+    std::string err;
+
+    // device buffers
+    double *mass = nullptr, *gm = nullptr, *radius = nullptr, *density = nullptr, *cd = nullptr;
+    double *gs = nullptr, *ge = nullptr, *stop = nullptr;
+    int32_t *type = nullptr, *migt = nullptr;
+    double *y0 = nullptr, *y = nullptr, *acc0 = nullptr, *yscale = nullptr, *error = nullptr;
+    double *fst = nullptr, *cin = nullptr, *cout = nullptr, *aos = nullptr;
+    double4 *J = nullptr;
+    double *part = nullptr, *nnr2 = nullptr;
+    int32_t *nnidx = nullptr;
+    double *ind = nullptr;
+    double *rm3 = nullptr, *nnd = nullptr;
+    int32_t *nni = nullptr;
+    double *migc = nullptr, *dragc = nullptr;
+    double *errblk = nullptr, *scal = nullptr, *coef = nullptr, *integ = nullptr;
+    double *h_scal = nullptr;   // pinned: [0] h, [1] t, [2] error max
+    size_t part_slots = 0;
+    std::vector<void *> allocs;
+
+    int fail(int code, const char *fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPCHECK(ctx, call)                                                                    \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return (ctx)->fail(SOL_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,                \
+                               hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+namespace {
+
+template <typename T>
+int dalloc(sol_context *c, T **p, size_t count)
+{
+    void *q = nullptr;
+    HIPCHECK(c, hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    HIPCHECK(c, hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(T)));
+    c->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return SOL_OK;
+}
+
+int n_pl_spl(const sol_nbodies &nb) { return nb.super_planetesimal + nb.planetesimal; }
+
+// Source range ends of the two astrocentric sink classes
+// (Solaris/Acceleration.cpp:285-289).
+void source_ends(const sol_context *c, int &ea, int &eb)
+{
+    ea = c->end_massive >= 0 ? c->end_massive : c->n_massive + c->nb.super_planetesimal;
+    eb = c->end_other >= 0 ? c->end_other : c->n_massive;
+}
+
+// Builds the pair-sum work plan for the context's sink classes.
+void build_plan(sol_context *c)
+{
+    PairPlan p{};
+    struct S { int sb, se, jb, je; } segs[2];
+    int ns = 0;
+    if (c->barycentric) {
+        segs[ns++] = {0, c->n_massive, 0, c->n_massive};
+        segs[ns++] = {c->n_massive, c->n, 0, c->n_massive};
+    } else {
+        int ea, eb;
+        source_ends(c, ea, eb);
+        segs[ns++] = {1, c->n_massive, 1, ea};
+        segs[ns++] = {c->n_massive, c->n, 1, eb};
+    }
+    int64_t items = 0;
+    int tiles = 0;
+    double inter = 0.0;
+    p.nseg = 0;
+    p.chunk = c->chunk;
+    for (int k = 0; k < ns; ++k) {
+        const S &s = segs[k];
+        const int nsink = s.se - s.sb, nsrc = s.je - s.jb;
+        if (nsink <= 0) continue;
+        // interactions: pair terms (minus self where the sink is a source) + Kepler (AC)
+        for (int pass = 0; pass < 1; ++pass) {
+            const int lo = std::max(s.sb, s.jb), hi = std::min(s.se, s.je);
+            const double self = hi > lo ? (double)(hi - lo) : 0.0;
+            inter += (double)nsink * (double)std::max(nsrc, 0) - self + (c->barycentric ? 0.0 : (double)nsink);
+        }
+        if (nsrc <= 0) continue;
+        PairSeg &g = p.seg[p.nseg++];
+        g.sink_begin = s.sb; g.sink_end = s.se;
+        g.src_begin = s.jb; g.src_end = s.je;
+        g.ntiles = (nsink + kWave - 1) / kWave;
+        g.nchunks = (nsrc + c->chunk - 1) / c->chunk;
+        g.tile_begin = tiles;
+        g.item_begin = items;
+        tiles += g.ntiles;
+        items += (int64_t)g.ntiles * g.nchunks;
+    }
+    p.ntiles = tiles;
+    p.total_items = items;
+    const int64_t wmax = (int64_t)c->cu_count * 4 * c->waves_per_simd;
+    p.nwaves = (int)std::max<int64_t>(1, std::min<int64_t>(wmax, items));
+    // slots per wave: the largest number of tiles one wave's range touches
+    auto tile_of = [&](int64_t x) {
+        const int s = (p.nseg > 1 && x >= p.seg[1].item_begin) ? 1 : 0;
+        return p.seg[s].tile_begin + (int)((x - p.seg[s].item_begin) / p.seg[s].nchunks);
+    };
+    int spw = 1;
+    if (items > 0) {
+        for (int64_t w = 0; w < p.nwaves; ++w) {
+            const int64_t lo = (w * items) / p.nwaves, hi = ((w + 1) * items) / p.nwaves;
+            if (hi > lo) spw = std::max(spw, tile_of(hi - 1) - tile_of(lo) + 1);
+        }
+    }
+    p.slots_per_wave = spw;
+    c->plan = p;
+    c->interactions_per_eval = inter;
+}
+
+int ensure_partials(sol_context *c)
+{
+    const size_t need = (size_t)c->plan.nwaves * c->plan.slots_per_wave;
+    if (need <= c->part_slots) return SOL_OK;
+    if (c->part) { (void)hipFree(c->part); (void)hipFree(c->nnr2); (void)hipFree(c->nnidx); }
+    c->part = nullptr; c->nnr2 = nullptr; c->nnidx = nullptr;
+    HIPCHECK(c, hipMalloc(&c->part, need * 3 * kWave * sizeof(double)));
+    HIPCHECK(c, hipMalloc(&c->nnr2, need * kWave * sizeof(double)));
+    HIPCHECK(c, hipMalloc(&c->nnidx, need * kWave * sizeof(int32_t)));
+    c->part_slots = need;
+    return SOL_OK;
+}
+
+GasDev to_dev(const sol_gas_component &g)
+{
+    GasDev d{};
+    d.alpha = g.alpha; d.mmw = g.mean_molecular_weight; d.particle_diameter = g.particle_diameter;
+    d.type = g.type; d.time_scale = g.time_scale; d.t0 = g.t0; d.t1 = g.t1; d.inner_edge = g.inner_edge;
+    d.eta_c = g.eta.c; d.eta_i = g.eta.index; d.tau_c = g.tau.c; d.tau_i = g.tau.index;
+    d.sh_c = g.scale_height.c; d.sh_i = g.scale_height.index; d.rho_c = g.density.c; d.rho_i = g.density.index;
+    d.mfp_c = g.mean_free_path.c; d.mfp_i = g.mean_free_path.index;
+    return d;
+}
+
+// Epilogue arguments common to all stages.
+EpiArgs base_args(sol_context *c)
+{
+    EpiArgs a{};
+    a.plan = c->plan;
+    a.part = c->part; a.nnr2 = c->nnr2; a.nnidx = c->nnidx;
+    a.n = c->n; a.np = c->np; a.nblk = c->nblk; a.barycentric = c->barycentric;
+    a.n_massive = c->n_massive; a.n_central = c->nb.central_body; a.n_giant = c->nb.giant_planet;
+    source_ends(c, a.src_end_a, a.src_end_b);
+    a.drag_lo = c->n_massive; a.drag_hi = c->n_massive + n_pl_spl(c->nb);
+    const bool gas_live = c->has_gas && !c->gas_dead;
+    a.gas_on = gas_live ? 1 : 0;
+    a.drag_eval = c->eval_drag;
+    a.mig_on = gas_live ? 1 : 0;
+    a.J = c->J;
+    a.mass = c->mass; a.radius = c->radius; a.density = c->density; a.cd = c->cd; a.gs = c->gs; a.ge = c->ge;
+    a.migt = c->migt; a.migc = c->migc; a.dragc = c->dragc;
+    a.gas = c->gasdev;
+    a.rm3_out = c->rm3; a.nni_out = c->nni; a.nnd_out = c->nnd;
+    a.y0 = c->y0; a.acc0 = c->acc0; a.fst = c->fst;
+    a.yscale = c->yscale; a.y_out = c->y; a.err_out = c->error; a.errblk = c->errblk;
+    a.scal = c->scal; a.coef = c->coef;
+    return a;
+}
+
+double *fslot(sol_context *c, int s) { return c->fst + (size_t)(s - 1) * 6 * c->np; }
+double *ind_buf(sol_context *c, int parity) { return c->ind + (size_t)(parity & 1) * c->nblk * 6; }
+
+int upload_scalars(sol_context *c, double h, double t)
+{
+    c->h_scal[0] = h;
+    c->h_scal[1] = t;
+    HIPCHECK(c, hipMemcpyAsync(c->scal, c->h_scal, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return SOL_OK;
+}
+
+// One full force evaluation of the state `st` (6 x NP) at time t, the
+// reference's Acceleration::Compute, with the derivative in `out`.
+// kind: EPI_COMPUTE or EPI_STAGE0 (the Driver's evaluation at y0; also
+// writes yscale for RKF78 with the trial h).
+int evaluate_at(sol_context *c, const double *st, double t, double h, double *out, int kind, int integ,
+                bool last, bool want_yscale)
+{
+    int r = upload_scalars(c, h, t);
+    if (r) return r;
+    int ea, eb;
+    source_ends(c, ea, eb);
+    launch_prep(c->stream, c->n, c->np, st, c->gm, c->J, ind_buf(c, 0), c->barycentric, ea, eb, nullptr);
+    const bool gas_live = c->has_gas && !c->gas_dead;
+    if (gas_live && (c->eval_mig1 || c->eval_mig2)) {
+        const int do1 = c->eval_mig1 && c->nb.proto_planet > 0;
+        const int do2 = c->eval_mig2 && c->nb.giant_planet > 0;
+        if (do1 || do2)
+            launch_migration(c->stream, c->nb.central_body, c->nb.giant_planet, c->n_massive, c->np, st, c->mass,
+                             c->stop, c->migt, c->migc, c->gasdev, c->barycentric, t, do1, do2);
+    }
+    launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
+                 c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+    EpiArgs a = base_args(c);
+    a.kind = kind;
+    a.integ = integ;
+    a.stage = 0;
+    a.last = last ? 1 : 0;
+    a.want_nn = last ? 1 : 0;
+    a.vel = st + 3 * (size_t)c->np;
+    a.ind_in = ind_buf(c, 0);
+    a.fout = out;
+    a.yscale_out = want_yscale ? c->yscale : nullptr;
+    launch_epilogue(c->stream, a);
+    HIPCHECK(c, hipGetLastError());
+    if (c->barycentric && c->has_gas && !c->gas_dead && host_reduction_factor(c->gas, t) < 1.0e-5) {
+        // Solaris/Acceleration.cpp:151-160: the gas is gone for good
+        c->gas_dead = true;
+        c->eval_drag = c->eval_mig1 = c->eval_mig2 = 0;
+    }
+    return SOL_OK;
+}
+
+int integ_kind(int32_t integrator)
+{
+    switch (integrator) {
+    case SOL_RUNGE_KUTTA_FEHLBERG78: return INTEG_RKF78;
+    case SOL_RUNGE_KUTTA4: return INTEG_RK4;
+    case SOL_DORMAND_PRINCE: return INTEG_DP;
+    default: return -1;
+    }
+}
+
+int n_stages(int ik) { return ik == INTEG_RKF78 ? 12 : ik == INTEG_RK4 ? 3 : 8; }
+
+// Stage evaluations 1..S of one Step from Y0 and ACCEL at (t, h) into Y,
+// ERROR and the error maximum (left in c->h_scal[2] after the sync when
+// read_err is set).
+int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double *errmax_out)
+{
+    int r = upload_scalars(c, h, t);
+    if (r) return r;
+    const int S = n_stages(ik);
+    EpiArgs a = base_args(c);
+    a.integ = ik;
+    // stage 1 state from y0 and f0
+    a.stage = 0;
+    a.fst_next = fslot(c, 1);
+    a.ind_out = ind_buf(c, 1);
+    launch_combine_first(c->stream, a);
+    for (int s = 1; s <= S; ++s) {
+        const bool last = (s == S);
+        launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
+                     c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+        EpiArgs e = a;
+        e.stage = s;
+        e.kind = last ? EPI_FINAL : EPI_MID;
+        e.last = last ? 1 : 0;
+        e.want_nn = last ? 1 : 0;
+        e.vel = fslot(c, s);
+        e.fout = fslot(c, s);
+        e.ind_in = ind_buf(c, s);
+        e.ind_out = ind_buf(c, s + 1);
+        e.fst_next = last ? nullptr : fslot(c, s + 1);
+        launch_epilogue(c->stream, e);
+    }
+    launch_errmax_final(c->stream, c->errblk, c->nblk, c->scal + 2);
+    HIPCHECK(c, hipGetLastError());
+    if (read_err) {
+        HIPCHECK(c, hipMemcpyAsync(c->h_scal + 2, c->scal + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (errmax_out) *errmax_out = c->h_scal[2];
+    }
+    return SOL_OK;
+}
+
+int driver_rkf78(sol_context *c, double eps, sol_timeline *tl, double *body_h, int64_t *attempts)
+{
+    const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = 1.89e-4;
+    const double t0 = tl->time;
+    double h = tl->hNext;
+    c->eval_drag = c->eval_mig1 = c->eval_mig2 = 1;
+    int r = evaluate_at(c, c->y0, t0, h, c->acc0, EPI_STAGE0, INTEG_RKF78, false, true);
+    if (r) return r;
+    c->eval_mig1 = c->eval_mig2 = 0;
+    double errmax = 0.0;
+    for (;;) {
+        double raw = 0.0;
+        r = run_stages(c, INTEG_RKF78, t0, h, true, &raw);
+        if (r) return r;
+        if (attempts) ++*attempts;
+        errmax = raw / eps;
+        if (errmax < 1.0) { tl->hDid = h; break; }
+        const double htemp = SAFETY * h * std::pow(errmax, PSHRNK);
+        h = std::fabs(htemp) > std::fabs(0.1 * h) ? htemp : 0.1 * h;
+        if (tl->time + h == tl->time) {
+            if (body_h) *body_h = h;
+            return c->fail(SOL_ERR_UNDERFLOW, "Stepsize-underflow occurred during Runge-Kutta-Fehlberg7(8) step!");
+        }
+    }
+    tl->time += tl->hDid;
+    tl->hNext = errmax > ERRCON ? (SAFETY * h * std::pow(errmax, PGROW)) : (5.0 * h);
+    if (body_h) *body_h = tl->hNext;
+    std::swap(c->y0, c->y);
+    return SOL_OK;
+}
+
+int driver_rk4(sol_context *c, sol_timeline *tl, double *body_h, int64_t *attempts)
+{
+    const double t0 = tl->time, h = tl->hNext;
+    c->eval_drag = c->eval_mig1 = c->eval_mig2 = 1;
+    int r = evaluate_at(c, c->y0, t0, h, c->acc0, EPI_STAGE0, INTEG_RK4, false, false);
+    if (r) return r;
+    c->eval_mig1 = c->eval_mig2 = 0;
+    r = run_stages(c, INTEG_RK4, t0, h, false, nullptr);
+    if (r) return r;
+    if (attempts) ++*attempts;
+    tl->hDid = h;
+    tl->time += tl->hDid;
+    tl->hNext = h;
+    if (body_h) *body_h = h;
+    std::swap(c->y0, c->y);
+    return SOL_OK;
+}
+
+int driver_dp(sol_context *c, double eps, sol_timeline *tl, double *body_h, int64_t *attempts)
+{
+    const int maxIter = 10;
+    const double t0 = tl->time;
+    c->eval_drag = c->eval_mig1 = c->eval_mig2 = 1;
+    int r = evaluate_at(c, c->y0, t0, tl->hNext, c->acc0, EPI_STAGE0, INTEG_DP, false, false);
+    if (r) return r;
+    c->eval_mig1 = c->eval_mig2 = 0;
+    int iter = 0;
+    double errmax = 0.0, h = 0.0;
+    do {
+        iter++;
+        h = tl->hNext;
+        r = run_stages(c, INTEG_DP, t0, h, true, &errmax);
+        if (r) return r;
+        if (attempts) ++*attempts;
+        tl->hDid = h;
+        tl->hNext = errmax < 1.0e-20 ? 2.0 * h : 0.9 * h * std::pow(eps / errmax, 1.0 / 7.0);
+    } while (errmax > eps && iter <= maxIter);
+    if (body_h) *body_h = h;
+    if (iter > maxIter)
+        return c->fail(SOL_ERR_MAXITER,
+                       "An error occurred during Prince-Dormand driver: iteration number exceeded maxIter!");
+    tl->time += tl->hDid;
+    std::swap(c->y0, c->y);
+    return SOL_OK;
+}
+
+int check_ctx(sol_context *c)
+{
+    if (!c) return SOL_ERR_ARGUMENT;
+    return SOL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sol_version(void) { return "solaris_amd 0.1 (gfx950)"; }
+
+const char *sol_last_error(const sol_context *ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+int sol_create(sol_context **out, const sol_nbodies *nb, int32_t barycentric, int32_t device)
+{
+    if (!out || !nb) { g_create_error = "null argument"; return SOL_ERR_ARGUMENT; }
+    *out = nullptr;
+    const int sum = nb->central_body + nb->giant_planet + nb->rocky_planet + nb->proto_planet +
+                    nb->super_planetesimal + nb->planetesimal + nb->test_particle;
+    if (nb->total <= 0) { g_create_error = "host memory allocation"; return SOL_ERR_ALLOCATION; }
+    if (sum != nb->total || nb->central_body != 1) {
+        g_create_error = "body counts must add up to total, with exactly one central body";
+        return SOL_ERR_ARGUMENT;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_create_error = "no HIP device available";
+        return SOL_ERR_DEVICE;
+    }
+    if (device < 0 || device >= ndev) { g_create_error = "device index out of range"; return SOL_ERR_ARGUMENT; }
+    sol_context *c = new sol_context();
+    c->device = device;
+    c->nb = *nb;
+    c->n = nb->total;
+    c->np = (c->n + kBlock - 1) / kBlock * kBlock;
+    c->nblk = c->np / kBlock;
+    c->n_massive = nb->central_body + nb->giant_planet + nb->rocky_planet + nb->proto_planet;
+    c->barycentric = barycentric ? 1 : 0;
+    int r = SOL_OK;
+    auto bail = [&](int code) {
+        g_create_error = c->err;
+        sol_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) { c->err = "hipSetDevice failed"; return bail(SOL_ERR_DEVICE); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(SOL_ERR_DEVICE);
+    }
+    const size_t np = c->np;
+    if ((r = dalloc(c, &c->mass, np)) || (r = dalloc(c, &c->gm, np)) || (r = dalloc(c, &c->radius, np)) ||
+        (r = dalloc(c, &c->density, np)) || (r = dalloc(c, &c->cd, np)) || (r = dalloc(c, &c->gs, np)) ||
+        (r = dalloc(c, &c->ge, np)) || (r = dalloc(c, &c->stop, np)) || (r = dalloc(c, &c->type, np)) ||
+        (r = dalloc(c, &c->migt, np)) || (r = dalloc(c, &c->y0, 6 * np)) || (r = dalloc(c, &c->y, 6 * np)) ||
+        (r = dalloc(c, &c->acc0, 6 * np)) || (r = dalloc(c, &c->yscale, 6 * np)) ||
+        (r = dalloc(c, &c->error, 6 * np)) || (r = dalloc(c, &c->fst, 12 * 6 * np)) ||
+        (r = dalloc(c, &c->cin, 6 * np)) || (r = dalloc(c, &c->cout, 6 * np)) || (r = dalloc(c, &c->aos, 6 * np)) ||
+        (r = dalloc(c, &c->J, np)) || (r = dalloc(c, &c->ind, 2 * (size_t)c->nblk * 6)) ||
+        (r = dalloc(c, &c->rm3, np)) || (r = dalloc(c, &c->nnd, np)) || (r = dalloc(c, &c->nni, np)) ||
+        (r = dalloc(c, &c->migc, 3 * np)) || (r = dalloc(c, &c->dragc, 3 * np)) ||
+        (r = dalloc(c, &c->errblk, c->nblk)) || (r = dalloc(c, &c->scal, 8)) || (r = dalloc(c, &c->coef, 275)) ||
+        (r = dalloc(c, &c->integ, 11 * np)))
+        return bail(r);
+    if (hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        c->err = "hipHostMalloc failed";
+        return bail(SOL_ERR_DEVICE);
+    }
+    std::vector<double> coef;
+    fill_coefficients(coef);
+    if (hipMemcpy(c->coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        c->err = "coefficient upload failed";
+        return bail(SOL_ERR_DEVICE);
+    }
+    // indexOfNN starts at -1 (Solaris/Acceleration.cpp:253, :262)
+    std::vector<int32_t> minus1(np, -1);
+    if (hipMemcpy(c->nni, minus1.data(), np * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        c->err = "upload failed";
+        return bail(SOL_ERR_DEVICE);
+    }
+    build_plan(c);
+    if ((r = ensure_partials(c))) return bail(r);
+    *out = c;
+    return SOL_OK;
+}
+
+int sol_destroy(sol_context *c)
+{
+    if (!c) return SOL_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->part) (void)hipFree(c->part);
+    if (c->nnr2) (void)hipFree(c->nnr2);
+    if (c->nnidx) (void)hipFree(c->nnidx);
+    if (c->h_scal) (void)hipHostFree(c->h_scal);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SOL_OK;
+}
+
+int sol_set_bodies(sol_context *c, const sol_body_params *p)
+{
+    if (check_ctx(c) || !p) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t n = c->n;
+    std::vector<double> zero(n, 0.0), gm(n);
+    std::vector<int32_t> zi(n, 0);
+    auto up = [&](double *dst, const double *src) {
+        return hipMemcpy(dst, src ? src : zero.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    };
+    if (p->type) {
+        // the interaction ranges assume type-sorted bodies (Solaris/Simulator.cpp:449-495)
+        for (size_t i = 1; i < n; ++i)
+            if (p->type[i] < p->type[i - 1])
+                return c->fail(SOL_ERR_ARGUMENT, "bodies are not sorted by type (body %zu)", i);
+    }
+    for (size_t i = 0; i < n; ++i) gm[i] = kGauss2 * (p->mass ? p->mass[i] : 0.0);
+    HIPCHECK(c, up(c->mass, p->mass));
+    HIPCHECK(c, up(c->radius, p->radius));
+    HIPCHECK(c, up(c->density, p->density));
+    HIPCHECK(c, up(c->cd, p->cd));
+    HIPCHECK(c, up(c->gs, p->gamma_stokes));
+    HIPCHECK(c, up(c->ge, p->gamma_epstein));
+    HIPCHECK(c, up(c->stop, p->mig_stop_at));
+    HIPCHECK(c, hipMemcpy(c->gm, gm.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->type, p->type ? p->type : zi.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->migt, p->mig_type ? p->mig_type : zi.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    return SOL_OK;
+}
+
+int sol_set_nebula(sol_context *c, const sol_gas_component *g)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    c->has_gas = g != nullptr;
+    c->gas_dead = false;
+    if (g) { c->gas = *g; c->gasdev = to_dev(*g); }
+    HIPCHECK(c, hipMemsetAsync(c->migc, 0, 3 * (size_t)c->np * sizeof(double), c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->dragc, 0, 3 * (size_t)c->np * sizeof(double), c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return SOL_OK;
+}
+
+int sol_set_evaluate(sol_context *c, int32_t d, int32_t m1, int32_t m2)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    c->eval_drag = d != 0; c->eval_mig1 = m1 != 0; c->eval_mig2 = m2 != 0;
+    return SOL_OK;
+}
+
+int sol_get_evaluate(const sol_context *c, int32_t *d, int32_t *m1, int32_t *m2)
+{
+    if (!c) return SOL_ERR_ARGUMENT;
+    if (d) *d = c->eval_drag;
+    if (m1) *m1 = c->eval_mig1;
+    if (m2) *m2 = c->eval_mig2;
+    return SOL_OK;
+}
+
+int sol_set_precision(sol_context *c, int32_t precision, double softening)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    if (precision != SOL_PRECISION_FP64 && precision != SOL_PRECISION_MIXED)
+        return c->fail(SOL_ERR_ARGUMENT, "unknown precision %d", precision);
+    if (!(softening >= 0.0)) return c->fail(SOL_ERR_ARGUMENT, "softening must be >= 0");
+    if (precision == SOL_PRECISION_MIXED)
+        return c->fail(SOL_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    c->precision = precision;
+    c->softening = softening;
+    return SOL_OK;
+}
+
+int sol_set_source_ranges(sol_context *c, int32_t a, int32_t b)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    if (a > c->n || b > c->n) return c->fail(SOL_ERR_ARGUMENT, "source range beyond total");
+    c->end_massive = a;
+    c->end_other = b;
+    build_plan(c);
+    return ensure_partials(c);
+}
+
+int sol_compute(sol_context *c, double t, const double *yh, double *dydt)
+{
+    if (check_ctx(c) || !yh || !dydt) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t bytes = 6 * (size_t)c->n * sizeof(double);
+    HIPCHECK(c, hipMemcpyAsync(c->aos, yh, bytes, hipMemcpyHostToDevice, c->stream));
+    launch_aos_to_soa(c->stream, c->n, c->np, c->aos, c->cin);
+    int r = evaluate_at(c, c->cin, t, 0.0, c->cout, EPI_COMPUTE, INTEG_RKF78, true, false);
+    if (r) return r;
+    launch_soa_to_aos(c->stream, c->n, c->np, c->cout, c->aos);
+    HIPCHECK(c, hipMemcpyAsync(dydt, c->aos, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return SOL_OK;
+}
+
+int sol_get_side_outputs(sol_context *c, double *rm3, int32_t *nn, double *dist, int32_t *mig)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t n = c->n;
+    if (rm3) HIPCHECK(c, hipMemcpyAsync(rm3, c->rm3, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (nn) HIPCHECK(c, hipMemcpyAsync(nn, c->nni, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (dist) HIPCHECK(c, hipMemcpyAsync(dist, c->nnd, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (mig) HIPCHECK(c, hipMemcpyAsync(mig, c->migt, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return SOL_OK;
+}
+
+int sol_set_mig_type(sol_context *c, const int32_t *mig)
+{
+    if (check_ctx(c) || !mig) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipMemcpy(c->migt, mig, c->n * sizeof(int32_t), hipMemcpyHostToDevice));
+    return SOL_OK;
+}
+
+static int upload_soa(sol_context *c, const double *host, double *dst)
+{
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipMemcpyAsync(c->aos, host, 6 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_aos_to_soa(c->stream, c->n, c->np, c->aos, dst);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return SOL_OK;
+}
+
+int sol_set_state(sol_context *c, const double *y0)
+{
+    if (check_ctx(c) || !y0) return SOL_ERR_ARGUMENT;
+    return upload_soa(c, y0, c->y0);
+}
+
+int sol_set_accel(sol_context *c, const double *a)
+{
+    if (check_ctx(c) || !a) return SOL_ERR_ARGUMENT;
+    return upload_soa(c, a, c->acc0);
+}
+
+int sol_get_array(sol_context *c, int32_t which, double *out)
+{
+    if (check_ctx(c) || !out) return SOL_ERR_ARGUMENT;
+    const double *src = nullptr;
+    switch (which) {
+    case SOL_ARRAY_Y0: src = c->y0; break;
+    case SOL_ARRAY_Y: src = c->y; break;
+    case SOL_ARRAY_ACCEL: src = c->acc0; break;
+    case SOL_ARRAY_YSCALE: src = c->yscale; break;
+    case SOL_ARRAY_ERROR: src = c->error; break;
+    default: return c->fail(SOL_ERR_ARGUMENT, "unknown array %d", which);
+    }
+    HIPCHECK(c, hipSetDevice(c->device));
+    launch_soa_to_aos(c->stream, c->n, c->np, src, c->aos);
+    HIPCHECK(c, hipMemcpyAsync(out, c->aos, 6 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return SOL_OK;
+}
+
+int sol_driver(sol_context *c, int32_t integrator, double eps, sol_timeline *tl, double *body_h)
+{
+    if (check_ctx(c) || !tl) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    switch (integrator) {
+    case SOL_RUNGE_KUTTA_FEHLBERG78: return driver_rkf78(c, eps, tl, body_h, nullptr);
+    case SOL_RUNGE_KUTTA4: return driver_rk4(c, tl, body_h, nullptr);
+    case SOL_DORMAND_PRINCE: return driver_dp(c, eps, tl, body_h, nullptr);
+    default: return c->fail(SOL_ERR_UNSUPPORTED, "integrator %d is not provided", integrator);
+    }
+}
+
+int sol_step(sol_context *c, int32_t integrator, double time, double h)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    const int ik = integ_kind(integrator);
+    if (ik < 0) return c->fail(SOL_ERR_UNSUPPORTED, "integrator %d is not provided", integrator);
+    HIPCHECK(c, hipSetDevice(c->device));
+    int r = run_stages(c, ik, time, h, false, nullptr);
+    if (r) return r;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return SOL_OK;
+}
+
+int sol_error_max(sol_context *c, int32_t integrator, double eps, double *out)
+{
+    if (check_ctx(c) || !out) return SOL_ERR_ARGUMENT;
+    const size_t nv = 6 * (size_t)c->n;
+    std::vector<double> e(nv), s(nv);
+    int r = sol_get_array(c, SOL_ARRAY_ERROR, e.data());
+    if (r) return r;
+    if (integrator == SOL_DORMAND_PRINCE) {
+        double m = 0.0;
+        for (double v : e) if (std::fabs(v) > m) m = std::fabs(v);
+        *out = m;
+        return SOL_OK;
+    }
+    r = sol_get_array(c, SOL_ARRAY_YSCALE, s.data());
+    if (r) return r;
+    double m = 0.0;
+    for (size_t i = 0; i < nv; ++i) {
+        const double v = std::fabs(e[i] / s[i]);
+        if (v > m) m = v;
+    }
+    *out = m / eps;
+    return SOL_OK;
+}
+
+int sol_integrate(sol_context *c, int32_t integrator, double eps, sol_timeline *tl, int64_t max_steps,
+                  sol_run_stats *stats)
+{
+    if (check_ctx(c) || !tl) return SOL_ERR_ARGUMENT;
+    const int ik = integ_kind(integrator);
+    if (ik < 0) return c->fail(SOL_ERR_UNSUPPORTED, "integrator %d is not provided", integrator);
+    HIPCHECK(c, hipSetDevice(c->device));
+    hipEvent_t e0, e1;
+    HIPCHECK(c, hipEventCreate(&e0));
+    HIPCHECK(c, hipEventCreate(&e1));
+    HIPCHECK(c, hipEventRecord(e0, c->stream));
+    int64_t steps = 0, attempts = 0;
+    int r = SOL_OK;
+    for (;;) {
+        double bh = 0.0;
+        if (ik == INTEG_RKF78) r = driver_rkf78(c, eps, tl, &bh, &attempts);
+        else if (ik == INTEG_RK4) r = driver_rk4(c, tl, &bh, &attempts);
+        else r = driver_dp(c, eps, tl, &bh, &attempts);
+        if (r) break;
+        ++steps;
+        // Simulator::DecisionMaking, Solaris/Simulator.cpp:183-245
+        tl->elapsedTime += tl->hDid;
+        tl->lastSave += tl->hDid;
+        tl->lastNSteps += tl->hDid;
+        const double actual = 1000.0 * 365.25 * tl->millenium + tl->time;
+        bool stop = false;
+        if (std::fabs(actual) >= std::fabs(tl->length)) stop = true;
+        else if (c->n <= 1) stop = true;
+        else {
+            if (std::fabs(actual + tl->hNext) > std::fabs(tl->length)) tl->hNext = tl->length - actual;
+            if (std::fabs(tl->lastSave) >= std::fabs(tl->output)) tl->lastSave = 0.0;
+            if (std::fabs(tl->lastSave + tl->hNext) > std::fabs(tl->output)) tl->hNext = tl->output - tl->lastSave;
+        }
+        if (steps % 100 == 0) launch_flush_small(c->stream, c->n, c->np, c->y, c->y0);   // :159-162
+        if (stop) break;
+        if (max_steps > 0 && steps >= max_steps) break;
+    }
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (stats) {
+        const int per = n_stages(ik) + 0;
+        stats->accepted_steps = steps;
+        stats->attempts = attempts;
+        stats->force_evaluations = steps + attempts * per;
+        stats->interactions = (double)stats->force_evaluations * c->interactions_per_eval;
+        stats->device_ms = ms;
+    }
+    return r;
+}
+
+int sol_integrals(sol_context *c, double *out)
+{
+    if (check_ctx(c) || !out) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t np = c->np, n = c->n;
+    launch_integrals(c->stream, c->n, c->np, c->y0, c->mass, c->integ);
+    std::vector<double> h(11 * np), m(n);
+    HIPCHECK(c, hipMemcpyAsync(h.data(), c->integ, 11 * np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(m.data(), c->mass, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    // Calculate::Integrals (Solaris/Calculate.cpp:43-63), body order
+    double M = 0.0;
+    for (int i = 0; i < c->n_massive; ++i) M += m[i];
+    out[0] = M;
+    for (int k = 0; k < 6; ++k) {
+        double s = 0.0;
+        for (size_t i = 0; i < n; ++i) s += h[k * np + i];
+        out[1 + k] = s / M;
+    }
+    out[7] = std::sqrt(out[1] * out[1] + out[2] * out[2] + out[3] * out[3]);
+    out[8] = std::sqrt(out[4] * out[4] + out[5] * out[5] + out[6] * out[6]);
+    double cc[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k)
+        for (size_t i = 0; i < n; ++i) cc[k] += h[(6 + k) * np + i];
+    out[9] = cc[0]; out[10] = cc[1]; out[11] = cc[2];
+    out[12] = std::sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+    double ke = 0.0, pe = 0.0;
+    for (size_t i = 0; i < n; ++i) { ke += h[9 * np + i]; pe += h[10 * np + i]; }
+    pe *= 0.5 * kGauss2;
+    out[13] = ke; out[14] = pe; out[15] = ke - pe;
+    return SOL_OK;
+}
+
+int sol_interactions_per_evaluation(const sol_context *c, double *count)
+{
+    if (!c || !count) return SOL_ERR_ARGUMENT;
+    *count = c->interactions_per_eval;
+    return SOL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// pairs.hip -- direct-sum pair accelerations on gfx950.
+//
+// Replaces the j-loop of Acceleration::GravityAC (Solaris/Acceleration.cpp:
+// 294-317) and of GravityBC_* (:558-580, :604-627).  One wavefront owns 64
+// sinks (one per lane); the sources of a chunk are read as a wave-uniform
+// stream of 32-byte J records, which the compiler turns into scalar loads
+// (s_load_dwordx16: two bodies per load) so the pair arithmetic runs with
+// SGPR operands and no LDS or barrier.  The pair sum is
+//   sum_j k^2 m_j (r_j - r_i) / |r_j - r_i|^3
+// with |d|^-3 from v_rsq_f64 plus one second-order correction:
+//   y0 = rsq(r2), e = 1 - r2 y0^2, |d|^-3 = y0^3 (1 + 3/2 e + 15/8 e^2),
+// 16 FP64 VALU operations and one v_rsq_f64 per pair.  The indirect and
+// Kepler terms are added per sink in the epilogue (stages.hip).
+#include <float.h>
+
+#include "sol_internal.h"
+
+namespace sol {
+namespace {
+
+__device__ __forceinline__ void decode_item(const PairPlan &p, int64_t it, int &s, int &tile, int &chunk)
+{
+    s = (p.nseg > 1 && it >= p.seg[1].item_begin) ? 1 : 0;
+    const int64_t local = it - p.seg[s].item_begin;
+    tile = (int)(local / p.seg[s].nchunks);
+    chunk = (int)(local - (int64_t)tile * p.seg[s].nchunks);
+}
+
+// NN: track the nearest source (BodyData::indexOfNN/distanceOfNN).
+// BC: ties go to the larger index (the reference scans j downwards).
+// SOFT: Plummer softening eps2 on the force (not on the NN distance).
+template <bool NN, bool BC, bool SOFT>
+__global__ __launch_bounds__(256) void k_pairs_fp64(PairPlan plan, const double4 *__restrict__ J,
+                                                     double *__restrict__ part, double *__restrict__ nnr2,
+                                                     int32_t *__restrict__ nnidx, double eps2,
+                                                     double c15, double c1875)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (w >= plan.nwaves) return;
+    const int64_t lo = ((int64_t)w * plan.total_items) / plan.nwaves;
+    const int64_t hi = ((int64_t)(w + 1) * plan.total_items) / plan.nwaves;
+
+    int slot = 0, cur = -1;
+    int sb = 0, se = 0, i = 0;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    double best = DBL_MAX;
+    int bidx = -1;
+
+    auto flush = [&](int k) {
+        const int64_t base = ((int64_t)w * plan.slots_per_wave + k) * (3 * kWave);
+        part[base + lane] = ax;
+        part[base + kWave + lane] = ay;
+        part[base + 2 * kWave + lane] = az;
+        if (NN) {
+            const int64_t nb = ((int64_t)w * plan.slots_per_wave + k) * kWave;
+            nnr2[nb + lane] = best;
+            nnidx[nb + lane] = bidx;
+        }
+    };
+
+    for (int64_t it = lo; it < hi; ++it) {
+        int s, t, c;
+        decode_item(plan, it, s, t, c);
+        const PairSeg &sg = plan.seg[s];
+        const int g = sg.tile_begin + t;
+        if (g != cur) {
+            if (cur >= 0) { flush(slot); ++slot; }
+            cur = g;
+            ax = ay = az = 0.0;
+            best = DBL_MAX; bidx = -1;
+            sb = sg.sink_begin + t * kWave;
+            se = min(sb + kWave, sg.sink_end);
+            i = min(sb + lane, se - 1);      // spare lanes repeat the last sink
+            const double4 me = J[i];
+            xi = me.x; yi = me.y; zi = me.z;
+        }
+        const int jb = sg.src_begin + c * plan.chunk;
+        const int je = min(jb + plan.chunk, sg.src_end);
+        if (jb >= se || sb >= je) {
+            // the chunk holds none of this wave's sinks: no self pair
+#pragma unroll 4
+            for (int j = jb; j < je; ++j) {
+                const double4 q = J[j];
+                const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double rs = SOFT ? r2 + eps2 : r2;
+                const double y0 = __builtin_amdgcn_rsq(rs);
+                const double tt = y0 * y0;
+                const double e = __builtin_fma(-rs, tt, 1.0);
+                const double y3 = tt * y0;
+                const double p = __builtin_fma(c1875, e, c15);
+                const double qq = e * p;
+                const double mq = q.w * y3;
+                const double f = __builtin_fma(mq, qq, mq);
+                ax = __builtin_fma(f, dx, ax);
+                ay = __builtin_fma(f, dy, ay);
+                az = __builtin_fma(f, dz, az);
+                if (NN) {
+                    if (BC ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
+                }
+            }
+        } else {
+            for (int j = jb; j < je; ++j) {
+                const double4 q = J[j];
+                const bool self = (j == i);
+                const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double rs = self ? 1.0 : (SOFT ? r2 + eps2 : r2);
+                const double m = self ? 0.0 : q.w;
+                const double y0 = __builtin_amdgcn_rsq(rs);
+                const double tt = y0 * y0;
+                const double e = __builtin_fma(-rs, tt, 1.0);
+                const double y3 = tt * y0;
+                const double p = __builtin_fma(c1875, e, c15);
+                const double qq = e * p;
+                const double mq = m * y3;
+                const double f = __builtin_fma(mq, qq, mq);
+                ax = __builtin_fma(f, dx, ax);
+                ay = __builtin_fma(f, dy, ay);
+                az = __builtin_fma(f, dz, az);
+                if (NN && !self) {
+                    if (BC ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
+                }
+            }
+        }
+    }
+    if (cur >= 0) flush(slot);
+}
+
+template <bool NN, bool BC, bool SOFT>
+void launch_fp64(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
+                 double *nnr2, int32_t *nnidx, double eps2)
+{
+    const int blocks = (plan.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((k_pairs_fp64<NN, BC, SOFT>), dim3(blocks), dim3(kBlock), 0, st,
+                       plan, J, part, nnr2, nnidx, eps2, 1.5, 1.875);
+}
+
+}  // namespace
+
+void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
+                  double *nnr2, int32_t *nnidx, bool nn, bool bc, bool mixed, double eps2,
+                  const double *jlo)
+{
+    (void)jlo;
+    (void)mixed;
+    if (plan.total_items == 0) return;
+    const bool soft = eps2 != 0.0;
+#define SOL_PAIRS(NN, BC, SOFT) launch_fp64<NN, BC, SOFT>(st, plan, J, part, nnr2, nnidx, eps2)
+    if (nn) {
+        if (bc) { if (soft) SOL_PAIRS(true, true, true); else SOL_PAIRS(true, true, false); }
+        else    { if (soft) SOL_PAIRS(true, false, true); else SOL_PAIRS(true, false, false); }
+    } else {
+        if (bc) { if (soft) SOL_PAIRS(false, true, true); else SOL_PAIRS(false, true, false); }
+        else    { if (soft) SOL_PAIRS(false, false, true); else SOL_PAIRS(false, false, false); }
+    }
+#undef SOL_PAIRS
+}
+
+}  // namespace sol

@@ -1,0 +1,115 @@
+// sol_internal.h -- types shared by the HIP kernels and the host context.
+//
+// Device layout (one context, N bodies, NP = N rounded up to 256):
+//   state arrays (Y0, Y, ACCEL, YSCALE, ERROR, stage derivatives F_s) are
+//   structure-of-arrays: component c of body i at [c*NP + i], c = 0..5;
+//   J[i] = (x, y, z, k^2 m) of the positions being evaluated, one 32-byte
+//   record per body, read by the pair kernel as a wave-uniform stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/solaris_amd.h"
+
+namespace sol {
+
+constexpr double kGauss  = 1.720209895e-2;           // Solaris/Constants.h:28
+constexpr double kGauss2 = 2.959122082855911025e-4;  // Solaris/Constants.h:29
+constexpr int kBlock = 256;   // threads per block of the elementwise kernels
+constexpr int kWave = 64;     // CDNA wavefront
+constexpr int kMaxSeg = 2;    // sink classes per force evaluation
+
+// One sink class: sinks [sink_begin, sink_end) see sources
+// [src_begin, src_end) (minus themselves).  Sinks are cut into tiles of 64
+// (one wavefront) and sources into chunks; a work item is one
+// (tile, chunk) pair.
+struct PairSeg {
+    int32_t sink_begin, sink_end;
+    int32_t src_begin, src_end;
+    int32_t ntiles, nchunks;
+    int32_t tile_begin;       // global index of this segment's first tile
+    int32_t pad;
+    int64_t item_begin;       // global index of this segment's first item
+};
+
+// Work plan of one pair-sum launch.  The items are split into nwaves
+// contiguous ranges (wave w owns [w*T/W, (w+1)*T/W)), so every wave gets
+// the same number of items to within one.  A wave accumulates over
+// consecutive items of the same tile and writes one partial per tile it
+// touched into its slots; slots_per_wave bounds that count.
+struct PairPlan {
+    PairSeg seg[kMaxSeg];
+    int32_t nseg;
+    int32_t chunk;            // sources per item
+    int32_t nwaves;
+    int32_t slots_per_wave;
+    int64_t total_items;
+    int32_t ntiles;
+    int32_t pad;
+};
+
+// Gas disk parameters as the kernels read them (sol_gas_component).
+struct GasDev {
+    double alpha, mmw, particle_diameter;
+    int32_t type, pad;
+    double time_scale, t0, t1, inner_edge;
+    double eta_c, eta_i, tau_c, tau_i, sh_c, sh_i, rho_c, rho_i, mfp_c, mfp_i;
+};
+
+enum EpiKind { EPI_COMPUTE = 0, EPI_STAGE0 = 1, EPI_MID = 2, EPI_FINAL = 3 };
+enum IntegKind { INTEG_RKF78 = 0, INTEG_RK4 = 1, INTEG_DP = 2 };
+
+// Arguments of the per-stage epilogue (see stages.hip).
+struct EpiArgs {
+    PairPlan plan;
+    const double *part;       // pair partial sums, [slot][3][64]
+    const double *nnr2;       // nearest-neighbour partials, [slot][64]
+    const int32_t *nnidx;
+    int32_t n, np, nblk, barycentric;
+    int32_t n_massive, n_central, n_giant, src_end_a;
+    int32_t src_end_b, drag_lo, drag_hi, gas_on;
+    int32_t mig_on, kind, integ, stage;
+    int32_t last, want_nn, drag_eval, pad1;
+    const double4 *J;         // stage positions (and k^2 m); rewritten in place for the next stage
+    const double *vel;        // stage velocities (3 x NP)
+    const double *mass, *radius, *density, *cd, *gs, *ge;
+    const int32_t *migt;      // BodyData::migType on the device
+    const double *migc;       // cached migration accelerations (3 x NP)
+    double *dragc;            // cached gas-drag accelerations (3 x NP)
+    GasDev gas;
+    const double *ind_in;     // indirect-term block partials of this stage, [blk][2][3]
+    double *ind_out;          // ... of the next stage
+    double *fout;             // derivative of this stage (6 x NP)
+    double *rm3_out; int32_t *nni_out; double *nnd_out;
+    const double *y0;         // 6 x NP
+    const double *acc0;       // f0 (6 x NP)
+    const double *fst;        // stage derivatives, stage l >= 1 at fst + (l-1)*6*NP
+    double *fst_next;         // derivative slot of stage s+1 (its velocity part is written)
+    const double *yscale;
+    double *yscale_out, *y_out, *err_out, *errblk;
+    const double *scal;       // device scalars: [0] = h, [1] = t
+    const double *coef;       // integrator coefficient tables (context.cpp)
+};
+
+// Host-side launchers (pairs.hip / stages.hip).
+void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
+                  double *nnr2, int32_t *nnidx, bool nn, bool barycentric, bool mixed,
+                  double eps2, const double *jlo);
+void launch_prep(hipStream_t st, int n, int np, const double *state, const double *gm,
+                 double4 *J, double *ind_out, int barycentric, int src_end_a, int src_end_b,
+                 double *rm3_side);
+void launch_migration(hipStream_t st, int n_central, int n_giant, int n_massive, int np,
+                      const double *state, const double *mass, const double *stop,
+                      int32_t *migt, double *migc, const GasDev &gas, int barycentric,
+                      double t, int do_type1, int do_type2);
+void launch_epilogue(hipStream_t st, const EpiArgs &a);
+void launch_combine_first(hipStream_t st, const EpiArgs &a);
+void launch_errmax_final(hipStream_t st, const double *errblk, int nblk, double *out);
+void launch_aos_to_soa(hipStream_t st, int n, int np, const double *aos, double *soa);
+void launch_soa_to_aos(hipStream_t st, int n, int np, const double *soa, double *aos);
+void launch_flush_small(hipStream_t st, int n, int np, double *a, double *b);
+void launch_integrals(hipStream_t st, int n, int np, const double *y0, const double *mass,
+                      double *per_body /* 11 x NP */);
+
+}  // namespace sol

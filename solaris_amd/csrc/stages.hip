@@ -1,0 +1,693 @@
+// stages.hip -- per-body kernels around the pair sum.
+//
+//  k_prep        state -> J records (x, y, z, k^2 m) and the per-block sums
+//                of the indirect term k^2 m_j r_j / r_j^3 over each source
+//                range (Solaris/Acceleration.cpp:256-264, :314-316).
+//  k_migration   Type I / Type II migration at the Driver's first force
+//                evaluation into the per-body cache (Acceleration.cpp:424-528,
+//                :645-764), mutating migType on the device.
+//  k_epilogue    per body: reduce the pair partials, add the Kepler term and
+//                subtract the indirect sum (GravityAC, :268-326), add gas drag
+//                (GasDragAC, :331-422) and the cached migration (:190-242);
+//                store the stage derivative; then either form the next RK
+//                stage's state (the stage-combine loops of
+//                RungeKuttaFehlberg78::Step :170-234, RungeKutta4::Step
+//                :100-116, DormandPrince::Step2 :274-409) or the final state,
+//                error estimate and the per-block error maximum (:236-242,
+//                :252-262; DormandPrince.cpp:471-503).
+// All arithmetic is FP64.  Reductions use a fixed order, so results are
+// reproducible run to run.
+#include <float.h>
+
+#include "sol_internal.h"
+
+namespace sol {
+namespace {
+
+// Coefficient tables (device buffer written by the context, see
+// context.cpp): offsets into it.
+constexpr int kRkfD = 0;       // [13][12]
+constexpr int kRk4A = 156;     // [4][3]
+constexpr int kRk4B = 168;     // [4]
+constexpr int kRk4C = 172;     // [4]
+constexpr int kDpA = 176;      // [9][8]
+constexpr int kDpB = 248;      // [9]
+constexpr int kDpBdh = 257;    // [9]
+constexpr int kDpC = 266;      // [9]
+
+#define SQR(a) ((a) * (a))
+
+__device__ __forceinline__ double pl(double c, double idx, double x) { return c * pow(x, idx); }
+
+__device__ double reduction_factor(const GasDev &g, double t)
+{   // GasComponent::ReductionFactor, Solaris/GasComponent.cpp:36-61
+    switch (g.type) {
+    case SOL_GAS_CONSTANT: return 1.0;
+    case SOL_GAS_LINEAR:
+        if (t <= g.t0) return 1.0;
+        if (t > g.t0 && t <= g.t1) return 1.0 - (t - g.t0) / (g.t1 - g.t0);
+        return 0.0;
+    case SOL_GAS_EXPONENTIAL: return exp(-t / g.time_scale);
+    default: return 1.0;
+    }
+}
+
+// Gas drag on one body (GasDragAC, Solaris/Acceleration.cpp:342-400;
+// gas_velocity / gas_density_at / MeanThermalSpeed_CMU,
+// Solaris/GasComponent.cpp:97-157, :221-243).  mu = k^2 m_star.
+__device__ void gas_drag(const GasDev &g, double factor, double mu, double m_star,
+                         double x, double y, double z, double vx, double vy, double vz,
+                         double R, double rho_b, double cd, double gstokes, double gepstein,
+                         double &ox, double &oy, double &oz)
+{
+    const double r = sqrt(SQR(x) + SQR(y) + SQR(z));        // spherical r
+    const double rc = sqrt(SQR(x) + SQR(y));                // cylindrical r
+    // circular velocity, branch structure of GasComponent::circular_velocity
+    double cx = 0.0, cy = 0.0;
+    {
+        const double vc = sqrt(mu / rc);
+        if (x == 0.0 && y == 0.0) {
+        } else if (y == 0.0) {
+            cy = x > 0.0 ? vc : -vc;
+        } else if (x == 0.0) {
+            cx = y > 0.0 ? -vc : vc;
+        } else if (x >= y) {
+            const double p = y / x;
+            cy = x >= 0 ? vc / sqrt(1.0 + SQR(p)) : -vc / sqrt(1.0 + SQR(p));
+            cx = -cy * p;
+        } else {
+            const double p = x / y;
+            cx = y >= 0 ? -vc / sqrt(1.0 + SQR(p)) : vc / sqrt(1.0 + SQR(p));
+            cy = -cx * p;
+        }
+    }
+    const double sub = sqrt(1.0 - 2.0 * pl(g.eta_c, g.eta_i, rc));
+    const double ux = vx - cx * sub, uy = vy - cy * sub, uz = vz;
+    // gas density at (x, y, z)
+    double dens;
+    {
+        const double h = pl(g.sh_c, g.sh_i, rc);
+        const double arg = SQR(z / h);
+        if (g.inner_edge < rc) {
+            dens = pl(g.rho_c, g.rho_i, rc) * exp(-arg);
+        } else {
+            const double a = g.rho_c * pow(g.inner_edge, g.rho_i - 4.0);
+            dens = a * SQR(SQR(rc)) * exp(-arg);
+        }
+    }
+    const double rho = factor * dens;
+    const double lambda = pl(g.mfp_c, g.mfp_i, r);
+    // MeanThermalSpeed_CMU(m_star, r) calls Temperature_CMU(r, m_star)
+    auto vth_of = [&]() {
+        const double kb = 1.3806488e-23 * ((1.0 / 1.98911e30) * SQR(1.0 / 1.495978707e11)) / SQR(1.0 / 86400.0);
+        const double mp = 1.672621777e-27 * (1.0 / 1.98911e30);
+        const double cvth = sqrt((8.0 * kb) / (3.14159265358979323846 * g.mmw * mp));
+        const double ctp = kGauss2 * (1.0 / (kb / mp));
+        const double pt = 2.0 * g.sh_i - 3.0;
+        const double ct = SQR(g.sh_c) * r * g.mmw * ctp;
+        return cvth * sqrt(ct * pow(m_star, pt));
+    };
+    double C;
+    if (R <= 0.1 * lambda) {                                   // Epstein
+        C = gepstein * vth_of() * rho;
+    } else if (R >= 10.0 * lambda) {                           // Stokes
+        C = gstokes * sqrt(ux * ux + uy * uy + uz * uz) * rho;
+    } else {                                                    // transition
+        const double l1 = 0.1 * lambda, l2 = 10.0 * lambda;
+        const double ge = 1.0 / (rho_b * l1);
+        const double gsk = 3.0 / 8.0 * cd / (rho_b * l2);
+        const double vth = vth_of();
+        const double K = gsk * sqrt(ux * ux + uy * uy + uz * uz) / (ge * vth);
+        const double eta = l2 / l1;
+        const double kappa = log10(K) / log10(eta);
+        const double gamma = ge * vth * pow(l1, -kappa);
+        C = gamma * pow(R, kappa) * rho;
+    }
+    ox = -C * ux;
+    oy = -C * uy;
+    oz = -C * uz;
+}
+
+// Semi-major axis and eccentricity (Ephemeris::CalculateOrbitalElement,
+// Solaris/Ephemeris.cpp:10-41); false for an unbound orbit (a, e untouched).
+__device__ bool orbital_a_e(double mu, double x, double y, double z, double vx, double vy, double vz,
+                            double &a, double &e)
+{
+    const double v2 = SQR(vx) + SQR(vy) + SQR(vz);
+    const double h = v2 / 2.0 + (-mu / sqrt(SQR(x) + SQR(y) + SQR(z)));
+    if (h >= 0.0) return false;
+    const double cx = y * vz - z * vy, cy = z * vx - x * vz, cz = x * vy - y * vx;
+    double e2 = 1.0 + 2.0 * (SQR(cx) + SQR(cy) + SQR(cz)) * h / (mu * mu);
+    if (fabs(e2) < 1.0e-14) e2 = 0.0;
+    e = sqrt(e2);
+    a = -mu / (2.0 * h);
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_migration(int n_central, int n_giant, int n_massive, int np,
+                                                    const double *__restrict__ st, const double *__restrict__ mass,
+                                                    const double *__restrict__ stop, int32_t *__restrict__ migt,
+                                                    double *__restrict__ migc, GasDev g, int bc, double t,
+                                                    int do1, int do2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_central || i >= n_massive) return;
+    const bool is_giant = i < n_central + n_giant;
+    if (is_giant ? !do2 : !do1) return;
+    const int want = is_giant ? SOL_MIGRATION_TYPE_II : SOL_MIGRATION_TYPE_I;
+    if (migt[i] != want) return;
+    const double x = st[i], y = st[np + i], z = st[2 * np + i];
+    const double vx = st[3 * np + i], vy = st[4 * np + i], vz = st[5 * np + i];
+    const double factor = reduction_factor(g, t);
+    const double r2 = SQR(x) + SQR(y) + SQR(z);
+    const double r = sqrt(r2);
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (r <= stop[i]) {
+        migt[i] = SOL_MIGRATION_NO;
+    } else {
+        const double m = mass[i], mc = mass[0];
+        double a = 0.0, e = 0.0;
+        orbital_a_e(kGauss2 * (mc + m), x, y, z, vx, vy, vz, a, e);  // failure ignored, as the reference does
+        const double O = kGauss * sqrt((mc + m) / (a * a * a));
+        const double h = pl(g.sh_c, g.sh_i, r);
+        if (!is_giant) {
+            // MigrationTypeIAC, Solaris/Acceleration.cpp:445-481
+            const double mid = pl(g.rho_c, g.rho_i, r) * h * 2.50662827463100024161;
+            const double C = SQR(mc) / (m * mid * a * a);
+            const double ar = h / r, er = e * r;
+            double tm = 0.0;
+            if (e < 1.1 * h / r) {
+                const double Cm = 2.0 / (2.7 + 1.1 * fabs(g.rho_i)) / O;   // TypeIMigrationTime :766-777
+                const double er1 = er / (1.3 * h), er2 = er / (1.1 * h);
+                const double frac = (1.0 + er1 * er1 * er1 * er1 * er1) / (1.0 - er2 * er2 * er2 * er2);
+                tm = 1.0 / (Cm * C * SQR(ar) * frac);
+            }
+            const double Ce = 0.1 / (0.78 * O);                             // :779-789
+            const double fr = 1.0 + 0.25 * ((er / h) * (er / h) * (er / h));
+            double te = Ce * C * (ar * ar * ar * ar) * fr;
+            const double ti = 2.0 / te;
+            const double vr = x * vx + y * vy + z * vz;
+            te = 2.0 * vr / (r2 * te);
+            ox = -factor * (tm * vx + te * x);
+            oy = -factor * (tm * vy + te * y);
+            oz = -factor * (tm * vz + te * z + ti * vz);
+        } else {
+            // MigrationTypeIIAC / IIBC, :508-524, :746-760; TauNu :831-849
+            const double tau = (g.tau_i == 2) ? g.tau_c * SQR(r / h) / (g.alpha * O)
+                                              : g.tau_c * pow(r / h, g.tau_i) / (g.alpha * O);
+            const double c0 = bc ? tau : 1.0 / tau;
+            const double vr = vx * x + vy * y + vz * z;
+            const double c1 = vr / r2;
+            ox = -factor * (c0 * (0.5 * vx + 50 * (c1 * x)));
+            oy = -factor * (c0 * (0.5 * vy + 50 * (c1 * y)));
+            oz = -factor * (c0 * (0.5 * vz + 50 * (c1 * z) + vz));
+        }
+    }
+    migc[i] = ox;
+    migc[np + i] = oy;
+    migc[2 * np + i] = oz;
+}
+
+// ---- block reductions with a fixed order ------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Sums v[0..5] over the block; result valid in thread 0.
+__device__ void block_sum6(double v[6], double (*lds)[6])
+{
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lds[wid][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; ++k) {
+            double s = lds[0][k];
+            for (int q = 1; q < kBlock / kWave; ++q) s += lds[q][k];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Indirect-term contribution of body i at position p (AC): k^2 m r / r^3,
+// split by membership in the two source ranges [1, ea) and [1, eb).
+__device__ __forceinline__ void indirect_contrib(int i, int n, double x, double y, double z, double gm,
+                                                 int ea, int eb, double v[6])
+{
+    for (int k = 0; k < 6; ++k) v[k] = 0.0;
+    if (i >= 1 && i < n) {
+        const double r2 = SQR(x) + SQR(y) + SQR(z);
+        const double r = sqrt(r2);
+        const double rm3 = 1.0 / (r2 * r);
+        const double cx = gm * rm3 * x, cy = gm * rm3 * y, cz = gm * rm3 * z;
+        if (i < ea) { v[0] = cx; v[1] = cy; v[2] = cz; }
+        if (i < eb) { v[3] = cx; v[4] = cy; v[5] = cz; }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_prep(int n, int np, const double *__restrict__ st,
+                                               const double *__restrict__ gm, double4 *__restrict__ J,
+                                               double *__restrict__ ind_out, int bc, int ea, int eb)
+{
+    __shared__ double lds[kBlock / kWave][6];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
+    if (i < n) {
+        x = st[i]; y = st[np + i]; z = st[2 * np + i]; m = gm[i];
+        J[i] = make_double4(x, y, z, m);
+    } else if (i < np) {
+        J[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    if (bc) return;
+    double v[6];
+    indirect_contrib(i, n, x, y, z, m, ea, eb, v);
+    block_sum6(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) ind_out[blockIdx.x * 6 + k] = v[k];
+}
+
+// ---- pair-partial lookup ------------------------------------------------
+__device__ __forceinline__ int64_t wave_lo(const PairPlan &p, int64_t w)
+{
+    return (w * p.total_items) / p.nwaves;
+}
+
+__device__ __forceinline__ int64_t wave_of_item(const PairPlan &p, int64_t x)
+{
+    int64_t w = ((x + 1) * p.nwaves - 1) / p.total_items;
+    return w < p.nwaves - 1 ? w : p.nwaves - 1;
+}
+
+__device__ __forceinline__ int tile_of_item(const PairPlan &p, int64_t x)
+{
+    const int s = (p.nseg > 1 && x >= p.seg[1].item_begin) ? 1 : 0;
+    return p.seg[s].tile_begin + (int)((x - p.seg[s].item_begin) / p.seg[s].nchunks);
+}
+
+// Sum of the pair partials for sink i; false if i is in no sink class.
+__device__ bool gather_pairs(const EpiArgs &a, int i, double &px, double &py, double &pz,
+                             double &best, int &bidx)
+{
+    const PairPlan &p = a.plan;
+    px = py = pz = 0.0;
+    best = DBL_MAX; bidx = -1;
+    for (int s = 0; s < p.nseg; ++s) {
+        const PairSeg &sg = p.seg[s];
+        if (i < sg.sink_begin || i >= sg.sink_end) continue;
+        const int t = (i - sg.sink_begin) / kWave;
+        const int lane = (i - sg.sink_begin) - t * kWave;
+        const int g = sg.tile_begin + t;
+        const int64_t A = sg.item_begin + (int64_t)t * sg.nchunks;
+        const int64_t B = A + sg.nchunks;
+        const int64_t w0 = wave_of_item(p, A), w1 = wave_of_item(p, B - 1);
+        for (int64_t w = w0; w <= w1; ++w) {
+            const int64_t lo = wave_lo(p, w), hi = wave_lo(p, w + 1);
+            if (lo >= hi || hi <= A || lo >= B) continue;
+            const int k = g - tile_of_item(p, lo);
+            const int64_t slot = w * p.slots_per_wave + k;
+            const double *q = a.part + slot * (3 * kWave);
+            px += q[lane];
+            py += q[kWave + lane];
+            pz += q[2 * kWave + lane];
+            if (a.want_nn) {
+                const double r2 = a.nnr2[slot * kWave + lane];
+                const int j = a.nnidx[slot * kWave + lane];
+                if (j >= 0) {
+                    const bool take = a.barycentric ? (r2 < best || (r2 == best && j > bidx))
+                                                    : (r2 < best || (r2 == best && (bidx < 0 || j < bidx)));
+                    if (take) { best = r2; bidx = j; }
+                }
+            }
+        }
+        return true;
+    }
+    return false;
+}
+
+// Stage derivative component c of stage l for body i.
+__device__ __forceinline__ double fetch_f(const EpiArgs &a, int l, int c, int i, int s_reg, const double f[6])
+{
+    if (l == s_reg) return f[c];
+    if (l == 0) return a.acc0[c * a.np + i];
+    return a.fst[(int64_t)(l - 1) * 6 * a.np + c * a.np + i];
+}
+
+// State of stage sn (the reference's yTemp before the sn-th evaluation).
+__device__ void combine_stage(const EpiArgs &a, const double *coef, int sn, int i, int s_reg,
+                              const double f[6], double h, double out[6])
+{
+    const int np = a.np;
+    if (a.integ == INTEG_DP) {
+        const double c = coef[kDpC + sn];
+        const double h2 = h * h;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double var = coef[kDpA + sn * 8 + 0] * fetch_f(a, 0, j + 3, i, s_reg, f);
+            for (int l = 1; l < sn; ++l) {
+                const double d = coef[kDpA + sn * 8 + l];
+                if (d == 0.0) continue;
+                var += d * fetch_f(a, l, j + 3, i, s_reg, f);
+            }
+            const double p0 = a.y0[j * np + i], v0 = a.y0[(j + 3) * np + i];
+            out[j] = p0 + c * h * v0 + h2 * (var);
+            out[j + 3] = v0 + h * (var);
+        }
+        return;
+    }
+    const double *D = coef + (a.integ == INTEG_RKF78 ? kRkfD + sn * 12 : kRk4A + sn * 3);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double acc = 0.0;
+        bool first = true;
+        for (int l = 0; l < sn; ++l) {
+            const double d = D[l];
+            if (d == 0.0) continue;
+            const double v = d * fetch_f(a, l, c, i, s_reg, f);
+            acc = first ? v : acc + v;
+            first = false;
+        }
+        out[c] = a.y0[c * np + i] + h * acc;
+    }
+}
+
+// Writes the next stage's state for body i: J record (positions and k^2 m),
+// velocity part of the next derivative slot, and returns the indirect-term
+// contribution in v.
+__device__ void emit_next(const EpiArgs &a, int i, const double y[6], double v[6])
+{
+    const double gm = kGauss2 * a.mass[i];
+    double4 *J = const_cast<double4 *>(a.J);
+    J[i] = make_double4(y[0], y[1], y[2], gm);
+    a.fst_next[i] = y[3];
+    a.fst_next[a.np + i] = y[4];
+    a.fst_next[2 * a.np + i] = y[5];
+    if (!a.barycentric) indirect_contrib(i, a.n, y[0], y[1], y[2], gm, a.src_end_a, a.src_end_b, v);
+}
+
+__global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
+{
+    const double *__restrict__ coef = a.coef;
+    __shared__ double ind_sum[6];
+    __shared__ double lds[kBlock / kWave][6];
+    __shared__ double red[kBlock / kWave];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int np = a.np;
+    const double h = a.scal[0], t = a.scal[1];
+
+    // sum of the indirect-term block partials of this stage (AC only)
+    if (!a.barycentric && threadIdx.x < kWave) {
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        for (int b = threadIdx.x; b < a.nblk; b += kWave)
+            for (int k = 0; k < 6; ++k) v[k] += a.ind_in[b * 6 + k];
+        for (int k = 0; k < 6; ++k) v[k] = wave_sum(v[k]);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 6; ++k) ind_sum[k] = v[k];
+    }
+    __syncthreads();
+
+    const bool valid = i < a.n;
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    if (valid) {
+        const double4 p = a.J[i];
+        const double vx = a.vel[i], vy = a.vel[np + i], vz = a.vel[2 * np + i];
+        double px, py, pz, best;
+        int bidx;
+        const bool sink = gather_pairs(a, i, px, py, pz, best, bidx);
+        double ax = 0.0, ay = 0.0, az = 0.0, rm3 = 0.0;
+        const bool star_ac = !a.barycentric && i == 0;
+        if (!star_ac) {
+            if (!a.barycentric) {
+                const double r2 = SQR(p.x) + SQR(p.y) + SQR(p.z);
+                const double r = sqrt(r2);
+                rm3 = 1.0 / (r2 * r);
+                const double mu = kGauss2 * (a.mass[0] + a.mass[i]);
+                const int end = (i < a.n_massive) ? a.src_end_a : a.src_end_b;
+                const int sgi = (i < a.n_massive) ? 0 : 3;
+                double ix = ind_sum[sgi], iy = ind_sum[sgi + 1], iz = ind_sum[sgi + 2];
+                if (i < end) {            // the sink is one of its own sources: drop j == i
+                    const double gm = kGauss2 * a.mass[i];
+                    ix -= gm * rm3 * p.x; iy -= gm * rm3 * p.y; iz -= gm * rm3 * p.z;
+                }
+                ax = -mu * rm3 * p.x;     // Kepler term first, pair sum added last
+                ay = -mu * rm3 * p.y;
+                az = -mu * rm3 * p.z;
+                if (sink) { ax += px - ix; ay += py - iy; az += pz - iz; }
+            } else if (sink) {
+                ax = px; ay = py; az = pz;
+            }
+            if (a.gas_on && i >= a.drag_lo && i < a.drag_hi) {
+                double dx, dy, dz;
+                if (a.drag_eval) {
+                    const double tt = (a.integ == INTEG_RK4) ? t + coef[kRk4C + a.stage] * h
+                                    : (a.integ == INTEG_DP) ? t + coef[kDpC + a.stage] * h : t;
+                    const double factor = reduction_factor(a.gas, tt);
+                    gas_drag(a.gas, factor, kGauss2 * a.mass[0], a.mass[0], p.x, p.y, p.z, vx, vy, vz,
+                             a.radius[i], a.density[i], a.cd[i], a.gs[i], a.ge[i], dx, dy, dz);
+                    a.dragc[i] = dx; a.dragc[np + i] = dy; a.dragc[2 * np + i] = dz;
+                } else {   // evaluateGasDrag off: the cached array is re-added (:182-198)
+                    dx = a.dragc[i]; dy = a.dragc[np + i]; dz = a.dragc[2 * np + i];
+                }
+                ax += dx; ay += dy; az += dz;
+            }
+            if (a.mig_on && i >= a.n_central && i < a.n_massive && a.migt[i] != SOL_MIGRATION_NO) {
+                ax += a.migc[i]; ay += a.migc[np + i]; az += a.migc[2 * np + i];
+            }
+            f[0] = vx; f[1] = vy; f[2] = vz; f[3] = ax; f[4] = ay; f[5] = az;
+        }
+        if (a.kind == EPI_COMPUTE || a.kind == EPI_STAGE0) {
+            for (int c = 0; c < 6; ++c) a.fout[c * np + i] = f[c];
+        } else {
+            for (int c = 3; c < 6; ++c) a.fout[c * np + i] = f[c];
+        }
+        if (a.last) {
+            if (!a.barycentric) a.rm3_out[i] = rm3;
+            a.nni_out[i] = (a.want_nn && bidx >= 0) ? bidx : -1;
+            a.nnd_out[i] = (a.want_nn && bidx >= 0) ? sqrt(best) : 0.0;
+        }
+    }
+
+    if (a.kind == EPI_STAGE0) {
+        // yscale = |y0| + |h f0| + TINY (RungeKuttaFehlberg78.cpp:87-89)
+        if (valid && a.yscale_out)
+            for (int c = 0; c < 6; ++c)
+                a.yscale_out[c * np + i] = fabs(a.y0[c * np + i]) + fabs(h * f[c]) + 1.0e-30;
+    } else if (a.kind == EPI_MID) {
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        if (valid) {
+            double y[6];
+            combine_stage(a, coef, a.stage + 1, i, a.stage, f, h, y);
+            emit_next(a, i, y, v);
+        }
+        if (!a.barycentric) {
+            block_sum6(v, lds);
+            if (threadIdx.x == 0)
+                for (int k = 0; k < 6; ++k) a.ind_out[blockIdx.x * 6 + k] = v[k];
+        }
+    } else if (a.kind == EPI_FINAL) {
+        double ratio = 0.0;
+        if (valid) {
+            const int s = a.stage;
+            double y[6], e[6];
+            if (a.integ == INTEG_RKF78) {
+                // RungeKuttaFehlberg78.cpp:236-242
+                const double b0 = 41.0 / 840.0, b5 = 34.0 / 105.0, b6 = 9.0 / 35.0, b8 = 9.0 / 280.0,
+                             b10 = 41.0 / 840.0;
+                for (int c = 0; c < 6; ++c) {
+                    const double F0 = fetch_f(a, 0, c, i, s, f), F5 = fetch_f(a, 5, c, i, s, f);
+                    const double F6 = fetch_f(a, 6, c, i, s, f), F7 = fetch_f(a, 7, c, i, s, f);
+                    const double F8 = fetch_f(a, 8, c, i, s, f), F9 = fetch_f(a, 9, c, i, s, f);
+                    const double F10 = fetch_f(a, 10, c, i, s, f), F11 = fetch_f(a, 11, c, i, s, f);
+                    const double F12 = fetch_f(a, 12, c, i, s, f);
+                    y[c] = a.y0[c * np + i] + h * (b0 * F0 + b5 * F5 + b6 * (F6 + F7) + b8 * (F8 + F9) + b10 * F10);
+                    e[c] = h * fabs(F0 + F10 - F11 - F12) * 41.0 / 840.0;
+                    ratio = fmax(ratio, fabs(e[c] / a.yscale[c * np + i]));
+                }
+            } else if (a.integ == INTEG_RK4) {
+                // RungeKutta4.cpp:119-121
+                const double *B = coef + kRk4B;
+                for (int c = 0; c < 6; ++c) {
+                    y[c] = a.y0[c * np + i] + h * (B[0] * fetch_f(a, 0, c, i, s, f) + B[1] * fetch_f(a, 1, c, i, s, f) +
+                                                   B[2] * fetch_f(a, 2, c, i, s, f) + B[3] * fetch_f(a, 3, c, i, s, f));
+                    e[c] = 0.0;
+                }
+            } else {
+                // DormandPrince.cpp:471-483
+                const double *B = coef + kDpB, *Bd = coef + kDpBdh;
+                const double h2 = h * h;
+                for (int j = 0; j < 3; ++j) {
+                    const double F0 = fetch_f(a, 0, j + 3, i, s, f), F4 = fetch_f(a, 4, j + 3, i, s, f);
+                    const double F5 = fetch_f(a, 5, j + 3, i, s, f), F6 = fetch_f(a, 6, j + 3, i, s, f);
+                    const double F7 = fetch_f(a, 7, j + 3, i, s, f), F8 = fetch_f(a, 8, j + 3, i, s, f);
+                    const double p0 = a.y0[j * np + i], v0 = a.y0[(j + 3) * np + i];
+                    y[j] = p0 + h * v0 + h2 * (B[0] * F0 + B[4] * F4 + B[5] * F5 + B[6] * F6 + B[7] * F7 + B[8] * F8);
+                    e[j] = h2 * fabs(F7 - F8) / 20.0;
+                    y[j + 3] = v0 + h * (Bd[0] * F0 + Bd[4] * F4 + Bd[5] * F5 + Bd[6] * F6 + Bd[7] * F7);
+                    e[j + 3] = 0.0;
+                    ratio = fmax(ratio, e[j]);
+                }
+            }
+            for (int c = 0; c < 6; ++c) {
+                a.y_out[c * np + i] = y[c];
+                a.err_out[c * np + i] = e[c];
+            }
+        }
+        ratio = wave_max(ratio);
+        const int wid = threadIdx.x / kWave;
+        if ((threadIdx.x & (kWave - 1)) == 0) red[wid] = ratio;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = red[0];
+            for (int q = 1; q < kBlock / kWave; ++q) m = fmax(m, red[q]);
+            a.errblk[blockIdx.x] = m;
+        }
+    }
+}
+
+// First stage state of an attempt, from y0 and f0 (all in memory).
+__global__ __launch_bounds__(256) void k_combine_first(EpiArgs a)
+{
+    const double *__restrict__ coef = a.coef;
+    __shared__ double lds[kBlock / kWave][6];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double h = a.scal[0];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < a.n) {
+        double y[6];
+        const double dummy[6] = {0, 0, 0, 0, 0, 0};
+        combine_stage(a, coef, 1, i, -1, dummy, h, y);
+        emit_next(a, i, y, v);
+    }
+    if (!a.barycentric) {
+        block_sum6(v, lds);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 6; ++k) a.ind_out[blockIdx.x * 6 + k] = v[k];
+    }
+}
+
+__global__ void k_errmax_final(const double *__restrict__ blk, int nblk, double *__restrict__ out)
+{
+    double m = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kWave) m = fmax(m, blk[b]);
+    m = wave_max(m);
+    if (threadIdx.x == 0) *out = m;
+}
+
+__global__ void k_aos_to_soa(int n, int np, const double *__restrict__ aos, double *__restrict__ soa)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    for (int c = 0; c < 6; ++c) soa[c * np + i] = (i < n) ? aos[6 * (int64_t)i + c] : 0.0;
+}
+
+__global__ void k_soa_to_aos(int n, int np, const double *__restrict__ soa, double *__restrict__ aos)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 6; ++c) aos[6 * (int64_t)i + c] = soa[c * np + i];
+}
+
+// Tools::CheckAgainstSmallestNumber on two state arrays (Solaris/Tools.cpp:39-46).
+__global__ void k_flush_small(int n, int np, double *__restrict__ a, double *__restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 6; ++c) {
+        if (fabs(a[c * np + i]) < 1.0e-50) a[c * np + i] = 0.0;
+        if (fabs(b[c * np + i]) < 1.0e-50) b[c * np + i] = 0.0;
+    }
+}
+
+// Per-body terms of Calculate::Integrals (Solaris/Calculate.cpp:74-172):
+// rows 0-5 m*y, 6-8 m (r x v), 9 kinetic 0.5 m v^2, 10 sum_j m_i m_j / r_ij.
+__global__ __launch_bounds__(256) void k_integrals(int n, int np, const double *__restrict__ y,
+                                                    const double *__restrict__ mass, double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = mass[i];
+    const double x = y[i], yy = y[np + i], z = y[2 * np + i];
+    const double vx = y[3 * np + i], vy = y[4 * np + i], vz = y[5 * np + i];
+    for (int c = 0; c < 6; ++c) out[c * np + i] = m * y[c * np + i];
+    out[6 * np + i] = m * (yy * vz - z * vy);
+    out[7 * np + i] = m * (z * vx - x * vz);
+    out[8 * np + i] = m * (x * vy - yy * vx);
+    out[9 * np + i] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    double pe = 0.0;
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        const double dx = y[j] - x, dy = y[np + j] - yy, dz = y[2 * np + j] - z;
+        pe += m * mass[j] / sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    out[10 * np + i] = pe;
+}
+
+inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+void launch_prep(hipStream_t st, int n, int np, const double *state, const double *gm, double4 *J,
+                 double *ind_out, int bc, int ea, int eb, double *rm3_side)
+{
+    (void)rm3_side;
+    hipLaunchKernelGGL(k_prep, dim3(np / kBlock), dim3(kBlock), 0, st, n, np, state, gm, J, ind_out, bc, ea, eb);
+}
+
+void launch_migration(hipStream_t st, int n_central, int n_giant, int n_massive, int np, const double *state,
+                      const double *mass, const double *stop, int32_t *migt, double *migc, const GasDev &gas,
+                      int bc, double t, int do1, int do2)
+{
+    if (n_massive <= 0) return;
+    hipLaunchKernelGGL(k_migration, dim3(nblocks(n_massive)), dim3(kBlock), 0, st, n_central, n_giant, n_massive,
+                       np, state, mass, stop, migt, migc, gas, bc, t, do1, do2);
+}
+
+void launch_epilogue(hipStream_t st, const EpiArgs &a)
+{
+    hipLaunchKernelGGL(k_epilogue, dim3(a.nblk), dim3(kBlock), 0, st, a);
+}
+
+void launch_combine_first(hipStream_t st, const EpiArgs &a)
+{
+    hipLaunchKernelGGL(k_combine_first, dim3(a.nblk), dim3(kBlock), 0, st, a);
+}
+
+void launch_errmax_final(hipStream_t st, const double *errblk, int nblk, double *out)
+{
+    hipLaunchKernelGGL(k_errmax_final, dim3(1), dim3(kWave), 0, st, errblk, nblk, out);
+}
+
+void launch_aos_to_soa(hipStream_t st, int n, int np, const double *aos, double *soa)
+{
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(nblocks(np)), dim3(kBlock), 0, st, n, np, aos, soa);
+}
+
+void launch_soa_to_aos(hipStream_t st, int n, int np, const double *soa, double *aos)
+{
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(nblocks(n)), dim3(kBlock), 0, st, n, np, soa, aos);
+}
+
+void launch_flush_small(hipStream_t st, int n, int np, double *a, double *b)
+{
+    hipLaunchKernelGGL(k_flush_small, dim3(nblocks(n)), dim3(kBlock), 0, st, n, np, a, b);
+}
+
+void launch_integrals(hipStream_t st, int n, int np, const double *y0, const double *mass, double *out)
+{
+    hipLaunchKernelGGL(k_integrals, dim3(nblocks(n)), dim3(kBlock), 0, st, n, np, y0, mass, out);
+}
+
+}  // namespace sol

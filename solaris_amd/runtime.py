@@ -1,0 +1,218 @@
+"""Python access to libsolaris_amd.so through its C ABI (ctypes).
+
+``Context`` mirrors the reference objects a caller drives: the force model
+(Acceleration::Compute, Solaris/Acceleration.h:19) and the integrator
+drivers (RungeKuttaFehlberg78 / RungeKutta4 / DormandPrince ::Driver, ::Step,
+::GetErrorMax) over a BodyData held on the GPU.  Every call goes to the HIP
+library; there is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsolaris_amd.so")
+
+FP64, MIXED = 0, 1
+Y0, Y, ACCEL, YSCALE, ERROR = 0, 1, 2, 3, 4
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int32)
+V = C.c_void_p
+
+_lib = None
+
+
+class SolarisError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_SIGS = {
+    "sol_create": (C.c_int, [C.POINTER(V), C.POINTER(_abi.NBodies), C.c_int32, C.c_int32]),
+    "sol_destroy": (C.c_int, [V]),
+    "sol_last_error": (C.c_char_p, [V]),
+    "sol_set_bodies": (C.c_int, [V, C.POINTER(_abi.BodyParams)]),
+    "sol_set_nebula": (C.c_int, [V, C.POINTER(_abi.GasComponent)]),
+    "sol_set_evaluate": (C.c_int, [V, C.c_int32, C.c_int32, C.c_int32]),
+    "sol_get_evaluate": (C.c_int, [V, I, I, I]),
+    "sol_set_precision": (C.c_int, [V, C.c_int32, C.c_double]),
+    "sol_set_source_ranges": (C.c_int, [V, C.c_int32, C.c_int32]),
+    "sol_compute": (C.c_int, [V, C.c_double, D, D]),
+    "sol_get_side_outputs": (C.c_int, [V, D, I, D, I]),
+    "sol_set_mig_type": (C.c_int, [V, I]),
+    "sol_set_state": (C.c_int, [V, D]),
+    "sol_get_array": (C.c_int, [V, C.c_int32, D]),
+    "sol_set_accel": (C.c_int, [V, D]),
+    "sol_driver": (C.c_int, [V, C.c_int32, C.c_double, C.POINTER(_abi.TimeLine), D]),
+    "sol_step": (C.c_int, [V, C.c_int32, C.c_double, C.c_double]),
+    "sol_error_max": (C.c_int, [V, C.c_int32, C.c_double, D]),
+    "sol_integrate": (C.c_int, [V, C.c_int32, C.c_double, C.POINTER(_abi.TimeLine), C.c_int64,
+                                C.POINTER(_abi.RunStats)]),
+    "sol_integrals": (C.c_int, [V, D]),
+    "sol_interactions_per_evaluation": (C.c_int, [V, D]),
+    "sol_version": (C.c_char_p, []),
+}
+
+
+def library() -> C.CDLL:
+    """Loads libsolaris_amd.so (built in-tree by solaris_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list:
+    return sorted(_SIGS)
+
+
+def _dp(a):
+    return a.ctypes.data_as(D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(I)
+
+
+class Context:
+    """One device-resident system (BodyData + Acceleration + integrators)."""
+
+    def __init__(self, scn, barycentric: bool = False, device: int = 0, precision: int = FP64):
+        self.lib = library()
+        self.n = scn.n
+        self._nb = _abi.nbodies(scn.counts)
+        h = V()
+        r = self.lib.sol_create(C.byref(h), C.byref(self._nb), int(barycentric), int(device))
+        if r:
+            raise SolarisError(r, self.lib.sol_last_error(None).decode())
+        self.h = h
+        self.set_bodies(scn)
+        self.set_nebula(scn.gas)
+        if precision != FP64 or scn.softening:
+            self.set_precision(precision, scn.softening)
+
+    def _check(self, r):
+        if r:
+            raise SolarisError(r, self.lib.sol_last_error(self.h).decode())
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sol_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- configuration --------------------------------------------------
+    def set_bodies(self, scn):
+        self._keep, self._bp = _abi.body_params(scn)
+        self._check(self.lib.sol_set_bodies(self.h, C.byref(self._bp)))
+
+    def set_nebula(self, gas):
+        self._gas = _abi.gas_component(gas) if gas is not None else None
+        self._check(self.lib.sol_set_nebula(self.h, C.byref(self._gas) if self._gas is not None else None))
+
+    def set_evaluate(self, drag, m1, m2):
+        self._check(self.lib.sol_set_evaluate(self.h, int(drag), int(m1), int(m2)))
+
+    def get_evaluate(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.sol_get_evaluate(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_precision(self, precision, softening=0.0):
+        self._check(self.lib.sol_set_precision(self.h, int(precision), float(softening)))
+
+    def set_softening(self, s):
+        self.set_precision(FP64, s)
+
+    def set_source_ranges(self, a, b):
+        self._check(self.lib.sol_set_source_ranges(self.h, int(a), int(b)))
+
+    def set_mig_type(self, mig):
+        m = np.ascontiguousarray(mig, dtype=np.int32)
+        self._check(self.lib.sol_set_mig_type(self.h, _ip(m)))
+
+    # -- force model ------------------------------------------------------
+    def compute(self, t, y):
+        y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(y)
+        self._check(self.lib.sol_compute(self.h, float(t), _dp(y), _dp(out)))
+        return out.reshape(-1, 6)
+
+    def side_outputs(self):
+        rm3 = np.zeros(self.n); nn = np.zeros(self.n, np.int32)
+        dist = np.zeros(self.n); mig = np.zeros(self.n, np.int32)
+        self._check(self.lib.sol_get_side_outputs(self.h, _dp(rm3), _ip(nn), _dp(dist), _ip(mig)))
+        return rm3, nn, dist, mig
+
+    # -- integrators --------------------------------------------------------
+    def set_state(self, y0):
+        y = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1)
+        self._check(self.lib.sol_set_state(self.h, _dp(y)))
+
+    def set_accel(self, a):
+        y = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        self._check(self.lib.sol_set_accel(self.h, _dp(y)))
+
+    def get_array(self, which):
+        out = np.zeros(6 * self.n)
+        self._check(self.lib.sol_get_array(self.h, int(which), _dp(out)))
+        return out.reshape(-1, 6)
+
+    def driver(self, integrator, eps, tl):
+        """Returns (status, BodyData::h); raises only on argument errors."""
+        bh = C.c_double(0.0)
+        r = self.lib.sol_driver(self.h, int(integrator), float(eps), C.byref(tl), C.byref(bh))
+        return r, bh.value
+
+    def step(self, integrator, time, h):
+        return self._check(self.lib.sol_step(self.h, int(integrator), float(time), float(h)))
+
+    def error_max(self, integrator, eps):
+        out = C.c_double(0.0)
+        self._check(self.lib.sol_error_max(self.h, int(integrator), float(eps), C.byref(out)))
+        return out.value
+
+    def integrate(self, integrator, eps, tl, max_steps=0):
+        st = _abi.RunStats()
+        r = self.lib.sol_integrate(self.h, int(integrator), float(eps), C.byref(tl), int(max_steps), C.byref(st))
+        return r, st
+
+    def integrals(self):
+        out = np.zeros(16)
+        self._check(self.lib.sol_integrals(self.h, _dp(out)))
+        return out
+
+    def interactions_per_evaluation(self):
+        out = C.c_double(0.0)
+        self._check(self.lib.sol_interactions_per_evaluation(self.h, C.byref(out)))
+        return out.value
+
+    def last_error(self):
+        return self.lib.sol_last_error(self.h).decode()
