@@ -1,0 +1,166 @@
+"""Benchmark: body-body interactions/s and RKF78 steps/s at N = 65,536.
+
+Workload: a 1 Msun star and 65,536 equal-mass (1e-10 Msun) self-gravitating
+bodies in a disk (a~U[0.5,5] AU, e~Rayleigh(0.02), i~Rayleigh(0.01)),
+adaptive RKF78 with tolerance 1e-10, state and interactions resident on
+the GPU.  A "step" is one accepted RKF78 step (13 force evaluations plus
+12 per rejected attempt, each 4.295e9 interactions).
+
+The JSON line reports the FP64 drop-in as `value`; the mixed-precision
+form, when the library provides it, in `mixed`.  `roofline` is the pair-sum
+kernel timed with HIP events on the library's stream over the timed
+region; `cpu_baseline` times the compiled reference (single thread, as the
+reference runs) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "body-body interactions/s & RKF78 steps/s, FP64 vs mixed precision, N=65536"
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 vector (= FP64 dense) peak
+FLOP_PER_INTERACTION = 20.0      # Nyland-Harris-Prins convention
+
+
+def run_gpu(scn, steps, warmup, precision):
+    from solaris_amd import _abi, runtime, scenarios as S
+    ctx = runtime.Context(scn, precision=precision)
+    ctx.set_state(scn.y0)
+    tl = _abi.timeline(0.0, scn.h_next, 1.0e12)
+    if warmup > 0:
+        r, _ = ctx.integrate(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl, max_steps=warmup)
+        if r:
+            raise RuntimeError(ctx.last_error())
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    r, st = ctx.integrate(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl, max_steps=steps)
+    t1 = time.perf_counter()
+    prof = ctx.profile(False)
+    if r:
+        raise RuntimeError(ctx.last_error())
+    wall = t1 - t0
+    out = dict(wall_s=wall, steps=int(st.accepted_steps), attempts=int(st.attempts),
+               evaluations=int(st.force_evaluations), interactions=st.interactions,
+               device_ms=st.device_ms, pair_launches=int(prof.pair_launches), pair_ms=prof.pair_ms,
+               pair_interactions=prof.pair_interactions, epilogue_ms=prof.epilogue_ms,
+               epilogue_launches=int(prof.epilogue_launches),
+               interactions_per_eval=ctx.interactions_per_evaluation())
+    ctx.close()
+    return out
+
+
+def cpu_baseline(seconds: float):
+    """The compiled reference (oracle/_ref, built from the reference's own
+    sources) or, where it is absent, the oracle restatement, single thread,
+    on the same disk with 8,192 bodies: Acceleration::Compute calls until
+    `seconds` of CPU time; interactions/s extrapolate to N = 65,536 as the
+    cost per interaction is independent of N for the direct sum."""
+    from oracle import cpu_ref
+    from solaris_amd import scenarios as S
+    kind = "reference" if cpu_ref.available("ref") else "port"
+    scn = S.make_c3(n=8192, softening=0.0)
+    sysobj = cpu_ref.Reference(scn) if kind == "reference" else cpu_ref.Oracle(scn)
+    ipe = scn.interactions_per_evaluation()
+    n_eval = 0
+    t0 = time.process_time()
+    while True:
+        sysobj.compute(0.0, scn.y0)
+        n_eval += 1
+        if time.process_time() - t0 >= seconds:
+            break
+    dt = time.process_time() - t0
+    rate = n_eval * ipe / dt
+    return dict(value=rate, unit="interactions/s", cores=1, kind=kind,
+                sample=f"{n_eval} Acceleration::Compute calls of the same disk at N=8,193 "
+                       f"({ipe:.3g} interactions each), {dt:.1f} s CPU",
+                steps_per_s_extrapolated=rate / (13.0 * 4294967296.0))
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("pairs_fp64_c3_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
+              "How the direct sum would shard over GPUs is described in DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    from solaris_amd import runtime, scenarios as S
+    scn = S.make_c3(n=args.n, softening=0.0)
+    print(f"bench: star + {args.n} bodies, RKF78 1e-10, FP64, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+    g = run_gpu(scn, args.steps, args.warmup, runtime.FP64)
+    value = g["interactions"] / g["wall_s"]
+    ms_per_step = 1000.0 * g["wall_s"] / max(g["steps"], 1)
+    pair_ms = g["pair_ms"] / max(g["pair_launches"], 1)
+    pairs_per_launch = g["pair_interactions"] / max(g["pair_launches"], 1)
+    achieved = FLOP_PER_INTERACTION * pairs_per_launch / (pair_ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": g["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 disk, seed 0x501A0003)",
+        "config": {"workload": f"star + {args.n} self-gravitating bodies, RKF78 tol 1e-10, device-resident",
+                   "n_bodies": args.n + 1, "integrator": "RKF78", "tolerance": 1e-10,
+                   "interactions_per_evaluation": g["interactions_per_eval"],
+                   "evaluations_per_accepted_step": g["evaluations"] / max(g["steps"], 1)},
+        "steps_per_s": g["steps"] / g["wall_s"],
+        "attempts": g["attempts"],
+        "roofline": {"bound": "fp64_vector", "kernel": "k_pairs_fp64",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(),
+                     "avg_launch_ms": pair_ms, "launches": g["pair_launches"],
+                     "pair_interactions_per_launch": pairs_per_launch,
+                     "flop_per_interaction": FLOP_PER_INTERACTION},
+        "epilogue_ms_per_launch": g["epilogue_ms"] / max(g["epilogue_launches"], 1),
+    }
+    if not args.no_mixed:
+        try:
+            m = run_gpu(S.make_c3(n=args.n), args.steps, args.warmup, runtime.MIXED)
+            line["mixed"] = {"value": m["interactions"] / m["wall_s"], "unit": "interactions/s",
+                             "steps_per_s": m["steps"] / m["wall_s"],
+                             "softening_au": 1e-5,
+                             "speedup_vs_fp64": (m["interactions"] / m["wall_s"]) / value,
+                             "pair_ms_per_launch": m["pair_ms"] / max(m["pair_launches"], 1)}
+        except runtime.SolarisError as e:
+            line["mixed"] = {"value": None, "note": str(e)}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        line["cpu_baseline"]["speedup_gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -241,6 +241,21 @@ int sol_integrate(sol_context *ctx, int32_t integrator, double epsilon,
  * length, kinetic, potential and total energy. */
 int sol_integrals(sol_context *ctx, double *out16);
 
+/* Kernel timing of the context's own stream. */
+typedef struct sol_kernel_profile {
+    int64_t pair_launches;     /* pair-sum kernel launches timed */
+    double  pair_ms;           /* their summed duration (HIP events) */
+    double  pair_interactions; /* pair terms they evaluated */
+    int64_t epilogue_launches;
+    double  epilogue_ms;
+} sol_kernel_profile;
+/* enable = 1 clears the totals and starts bracketing every pair-sum and
+ * epilogue launch with HIP events on the context's stream; enable = 0
+ * stops.  out (may be NULL) receives the totals so far (this call waits
+ * for the stream).  No counterpart in the reference, which has no
+ * profiling; it serves benchmark and roofline reporting. */
+int sol_profile(sol_context *ctx, int32_t enable, sol_kernel_profile *out);
+
 /* Interactions per force evaluation under the context's source ranges:
  * sum over sinks i >= 1 of (1 + |S(i) \ {i}|). */
 int sol_interactions_per_evaluation(const sol_context *ctx, double *count);

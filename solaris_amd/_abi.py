@@ -47,6 +47,12 @@ class RunStats(C.Structure):
                 ("device_ms", C.c_double)]
 
 
+class KernelProfile(C.Structure):
+    _fields_ = [("pair_launches", C.c_int64), ("pair_ms", C.c_double),
+                ("pair_interactions", C.c_double), ("epilogue_launches", C.c_int64),
+                ("epilogue_ms", C.c_double)]
+
+
 def nbodies(counts: dict) -> NBodies:
     nb = NBodies()
     for k, _ in NBodies._fields_:

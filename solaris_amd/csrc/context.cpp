@@ -134,6 +134,14 @@ struct sol_context {
     double *h_scal = nullptr;   // pinned: [0] h, [1] t, [2] error max
     size_t part_slots = 0;
     std::vector<void *> allocs;
+    // profiling (sol_profile)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Timed { hipEvent_t a, b; int kind; double inter; };
+    std::vector<Timed> timed;
+    size_t ev_used = 0;
+    sol_kernel_profile prof{};
+    double pair_inter_per_launch = 0.0;
 
     int fail(int code, const char *fmt, ...)
     {
@@ -221,6 +229,13 @@ void build_plan(sol_context *c)
     }
     p.ntiles = tiles;
     p.total_items = items;
+    double pairs = 0.0;
+    for (int k = 0; k < p.nseg; ++k) {
+        const PairSeg &g = p.seg[k];
+        const int lo = std::max(g.sink_begin, g.src_begin), hi = std::min(g.sink_end, g.src_end);
+        pairs += (double)(g.sink_end - g.sink_begin) * (double)(g.src_end - g.src_begin) - (hi > lo ? hi - lo : 0);
+    }
+    c->pair_inter_per_launch = pairs;
     const int64_t wmax = (int64_t)c->cu_count * 4 * c->waves_per_simd;
     p.nwaves = (int)std::max<int64_t>(1, std::min<int64_t>(wmax, items));
     // slots per wave: the largest number of tiles one wave's range touches
@@ -251,6 +266,50 @@ int ensure_partials(sol_context *c)
     HIPCHECK(c, hipMalloc(&c->nnidx, need * kWave * sizeof(int32_t)));
     c->part_slots = need;
     return SOL_OK;
+}
+
+// Event pair around a launch when profiling (kind 0: pair sum, 1: epilogue).
+hipEvent_t prof_event(sol_context *c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+struct ProfScope {
+    sol_context *c;
+    hipEvent_t a = nullptr;
+    int kind;
+    ProfScope(sol_context *c_, int kind_) : c(c_), kind(kind_)
+    {
+        if (c->profiling && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        hipEvent_t b = prof_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->timed.push_back({a, b, kind, kind == 0 ? c->pair_inter_per_launch : 0.0});
+    }
+};
+
+// Folds finished event pairs into the totals and recycles the events.
+void prof_collect(sol_context *c)
+{
+    if (c->timed.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &t : c->timed) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, t.a, t.b);
+        if (t.kind == 0) { c->prof.pair_launches++; c->prof.pair_ms += ms; c->prof.pair_interactions += t.inter; }
+        else { c->prof.epilogue_launches++; c->prof.epilogue_ms += ms; }
+    }
+    c->timed.clear();
+    c->ev_used = 0;
 }
 
 GasDev to_dev(const sol_gas_component &g)
@@ -320,8 +379,11 @@ int evaluate_at(sol_context *c, const double *st, double t, double h, double *ou
             launch_migration(c->stream, c->nb.central_body, c->nb.giant_planet, c->n_massive, c->np, st, c->mass,
                              c->stop, c->migt, c->migc, c->gasdev, c->barycentric, t, do1, do2);
     }
-    launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
-                 c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+    {
+        ProfScope ps(c, 0);
+        launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
+                     c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+    }
     EpiArgs a = base_args(c);
     a.kind = kind;
     a.integ = integ;
@@ -332,7 +394,10 @@ int evaluate_at(sol_context *c, const double *st, double t, double h, double *ou
     a.ind_in = ind_buf(c, 0);
     a.fout = out;
     a.yscale_out = want_yscale ? c->yscale : nullptr;
-    launch_epilogue(c->stream, a);
+    {
+        ProfScope ps(c, 1);
+        launch_epilogue(c->stream, a);
+    }
     HIPCHECK(c, hipGetLastError());
     if (c->barycentric && c->has_gas && !c->gas_dead && host_reduction_factor(c->gas, t) < 1.0e-5) {
         // Solaris/Acceleration.cpp:151-160: the gas is gone for good
@@ -371,8 +436,11 @@ int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double
     launch_combine_first(c->stream, a);
     for (int s = 1; s <= S; ++s) {
         const bool last = (s == S);
-        launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
-                     c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+        {
+            ProfScope ps(c, 0);
+            launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
+                         c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+        }
         EpiArgs e = a;
         e.stage = s;
         e.kind = last ? EPI_FINAL : EPI_MID;
@@ -383,7 +451,10 @@ int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double
         e.ind_in = ind_buf(c, s);
         e.ind_out = ind_buf(c, s + 1);
         e.fst_next = last ? nullptr : fslot(c, s + 1);
-        launch_epilogue(c->stream, e);
+        {
+            ProfScope ps(c, 1);
+            launch_epilogue(c->stream, e);
+        }
     }
     launch_errmax_final(c->stream, c->errblk, c->nblk, c->scal + 2);
     HIPCHECK(c, hipGetLastError());
@@ -575,6 +646,7 @@ int sol_destroy(sol_context *c)
     if (c->nnr2) (void)hipFree(c->nnr2);
     if (c->nnidx) (void)hipFree(c->nnidx);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SOL_OK;
@@ -870,6 +942,21 @@ int sol_integrals(sol_context *c, double *out)
     for (size_t i = 0; i < n; ++i) { ke += h[9 * np + i]; pe += h[10 * np + i]; }
     pe *= 0.5 * kGauss2;
     out[13] = ke; out[14] = pe; out[15] = ke - pe;
+    return SOL_OK;
+}
+
+int sol_profile(sol_context *c, int32_t enable, sol_kernel_profile *out)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    HIPCHECK(c, hipSetDevice(c->device));
+    prof_collect(c);
+    if (out) *out = c->prof;
+    if (enable) {
+        c->prof = sol_kernel_profile{};
+        c->profiling = true;
+    } else {
+        c->profiling = false;
+    }
     return SOL_OK;
 }
 

@@ -58,6 +58,7 @@ _SIGS = {
     "sol_integrals": (C.c_int, [V, D]),
     "sol_interactions_per_evaluation": (C.c_int, [V, D]),
     "sol_version": (C.c_char_p, []),
+    "sol_profile": (C.c_int, [V, C.c_int32, C.POINTER(_abi.KernelProfile)]),
 }
 
 
@@ -213,6 +214,13 @@ class Context:
         out = C.c_double(0.0)
         self._check(self.lib.sol_interactions_per_evaluation(self.h, C.byref(out)))
         return out.value
+
+    def profile(self, enable: bool):
+        """Starts/stops HIP-event timing of the pair-sum and epilogue
+        launches; returns the totals collected so far."""
+        out = _abi.KernelProfile()
+        self._check(self.lib.sol_profile(self.h, int(enable), C.byref(out)))
+        return out
 
     def last_error(self):
         return self.lib.sol_last_error(self.h).decode()
