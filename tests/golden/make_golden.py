@@ -1,0 +1,176 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Runs only in the development container, where oracle/Makefile has built
+oracle/_ref/libsolref.so from the reference's own sources.  Each fixture
+holds its complete input (body arrays, state, gas parameters as JSON) and
+the reference's outputs, so the tests never need the reference itself.
+
+Each fixture is produced in a fresh process: the reference keeps the first
+call's gas constants in function-level statics (Solaris/GasComponent.cpp:
+223-224, :240), so two gas configurations must not share a process.
+
+    python tests/golden/make_golden.py          # all fixtures
+    python tests/golden/make_golden.py NAME     # one fixture
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import cpu_ref  # noqa: E402
+from solaris_amd import _abi, scenarios as S  # noqa: E402
+
+METER_TO_AU = 1.0 / 1.495978707e11
+
+
+def scn_arrays(scn):
+    d = {k: np.asarray(getattr(scn, k)) for k in
+         ("mass", "radius", "density", "cd", "gamma_stokes", "gamma_epstein", "mig_stop_at",
+          "type", "mig_type", "y0")}
+    d["counts"] = json.dumps(scn.counts)
+    d["gas"] = json.dumps(scn.gas)
+    d["meta"] = json.dumps(dict(name=scn.name, h_next=scn.h_next, epsilon=scn.epsilon))
+    return d
+
+
+def drag_regimes():
+    """Planetesimals whose radii (1 mm .. 10 km) put them in the Epstein,
+    transition and Stokes regimes (the mean free path of the default law is
+    ~7 m at 1 AU), at positions that reach every branch of the circular
+    velocity, under an exponentially decaying gas."""
+    rng = S.SplitMix64(0x501A00D1)
+    n = 240
+    rho = 2.0 * S.GCM3_TO_SOLAR_AU3
+    R = np.exp(rng.uniform(n, np.log(1e-3), np.log(1e4))) * METER_TO_AU
+    m = rho * 4.0 / 3.0 * S.PI * R ** 3
+    a = rng.uniform(n, 0.3, 8.0)
+    e = rng.uniform(n, 0.0, 0.2)
+    inc = rng.uniform(n, 0.0, 0.2)
+    y = S._orbits(rng, n, 1.0, m, a, e, inc)
+    # exact axis and quadrant positions for the circular-velocity branches
+    y[0, :3] = (1.3, 0.0, 0.01)
+    y[1, :3] = (-1.7, 0.0, -0.02)
+    y[2, :3] = (0.0, 2.1, 0.0)
+    y[3, :3] = (0.0, -0.9, 0.0)
+    y[4, :3] = (-1.0, -2.0, 0.05)
+    y[5, :3] = (-2.0, -1.0, 0.0)
+    y[6, :3] = (0.04, 0.02, 0.0)        # inside the inner edge (10 solar radii)
+    cd = np.full(n, 0.44)
+    cd[10:20] = 0.0                      # gammaStokes = 0 branch of BodyListToBodyData
+    pls = dict(mass=m, type=S.PL, radius=R, density=np.full(n, rho), cd=cd, y0=y)
+    proto = dict(mass=np.array([3e-6]), type=S.PROTO, y0=S._orbits(rng, 1, 1.0, 3e-6, np.array([4.0]),
+                                                                   np.array([0.02]), np.array([0.01])))
+    gas = S.default_gas()
+    gas["type"] = 2                      # EXPONENTIAL
+    gas["time_scale"] = 1.0e5
+    return S._assemble("drag_regimes", S._counts(central_body=1, proto_planet=1, planetesimal=n),
+                       [S._star(), proto, pls], gas=gas)
+
+
+def migration_mix(bc=False):
+    """Type II giants and Type I rocky/proto planets, one of each stopped by
+    migStopAt on the first evaluation (Solaris/Acceleration.cpp:439-443,
+    :502-506), plus drag-free planetesimals."""
+    rng = S.SplitMix64(0x501A00D2)
+    gm = np.array([1e-3, 3e-4, 5e-5])
+    gas_giants = dict(mass=gm, type=S.GIANT, mig=S.MIG_II, y0=S._orbits(rng, 3, 1.0, gm, np.array([5.0, 9.0, 2.0]),
+                                                                      np.array([0.05, 0.02, 0.1]),
+                                                                      np.array([0.01, 0.02, 0.0])))
+    pm = np.array([1e-5, 2e-5, 4e-6])
+    protos = dict(mass=pm, type=S.PROTO, mig=S.MIG_I, y0=S._orbits(rng, 3, 1.0, pm, np.array([3.0, 12.0, 1.5]),
+                                                                   np.array([0.01, 0.2, 0.03]),
+                                                                   np.array([0.0, 0.03, 0.01])))
+    scn = S._assemble("migration_mix", S._counts(central_body=1, giant_planet=3, proto_planet=3),
+                      [S._star(), gas_giants, protos], gas=S.default_gas())
+    scn.mig_stop_at[3] = 3.0            # giant at a=2 AU stops
+    scn.mig_stop_at[6] = 2.0            # proto at a=1.5 AU stops
+    return scn
+
+
+def tp_only():
+    """The reference semantics with no sources for the other sinks:
+    planetesimals see only massive bodies; a system of a star and test
+    particles has pure Kepler accelerations."""
+    rng = S.SplitMix64(0x501A00D3)
+    n = 50
+    y = S._orbits(rng, n, 1.0, 0.0, rng.uniform(n, 0.5, 3.0), rng.uniform(n, 0, 0.3),
+                  rng.uniform(n, 0, 0.5), massless=True)
+    return S._assemble("tp_only", S._counts(central_body=1, test_particle=n),
+                       [S._star(), dict(mass=np.zeros(n), type=S.TP, y0=y)])
+
+
+def compute_fixture(name, scn, t=0.0, bc=False, evaluate=None):
+    ref = cpu_ref.Reference(scn, barycentric=bc)
+    if evaluate is not None:
+        ref.set_evaluate(*evaluate)
+    dydt = ref.compute(t, scn.y0)
+    rm3, nn, dist, mig = ref.side_outputs()
+    out = scn_arrays(scn)
+    out.update(t=t, barycentric=int(bc), dydt=dydt, rm3=rm3, index_nn=nn, distance_nn=dist, mig_type_after=mig)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+
+
+def driver_fixture(name, scn, integrator, eps, steps, h0):
+    ref = cpu_ref.Reference(scn)
+    ref.set_state(scn.y0)
+    tl = _abi.timeline(0.0, h0, 1e12)
+    hdid, hnext, times, bh = [], [], [], []
+    snaps = []
+    for k in range(steps):
+        r, b = ref.driver(integrator, eps, tl)
+        assert r == 0
+        hdid.append(tl.hDid); hnext.append(tl.hNext); times.append(tl.time); bh.append(b)
+        if (k + 1) % 10 == 0:
+            snaps.append(ref.get_array(0))
+    out = scn_arrays(scn)
+    rm3, nn, dist, mig = ref.side_outputs()
+    out.update(integrator=integrator, epsilon=eps, h0=h0, hdid=np.array(hdid), hnext=np.array(hnext),
+               time=np.array(times), body_h=np.array(bh), snapshots=np.array(snaps),
+               y0_final=ref.get_array(0), y_final=ref.get_array(1), accel_final=ref.get_array(2),
+               yscale_final=ref.get_array(3), error_final=ref.get_array(4), rm3=rm3, index_nn=nn,
+               distance_nn=dist, mig_type_after=mig, integrals_final=ref.integrals())
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+
+
+FIXTURES = {
+    "ref_c1_compute": lambda: compute_fixture("ref_c1_compute", S.make_c1(n_tp=128)),
+    "ref_c2_compute": lambda: compute_fixture("ref_c2_compute", S.make_c2(n_pl=256), t=10.0),
+    "ref_c3_subset_compute": lambda: compute_fixture("ref_c3_subset_compute", S.make_c3(n=1024, softening=0.0)),
+    "ref_drag_regimes": lambda: compute_fixture("ref_drag_regimes", drag_regimes(), t=3.0e4),
+    "ref_migration_ac": lambda: compute_fixture("ref_migration_ac", migration_mix(), t=100.0),
+    "ref_migration_bc": lambda: compute_fixture("ref_migration_bc", migration_mix(), t=100.0, bc=True),
+    "ref_bc_compute": lambda: compute_fixture("ref_bc_compute", S.make_c5(n_tp=96), bc=True),
+    "ref_tp_only": lambda: compute_fixture("ref_tp_only", tp_only()),
+    "ref_rkf78_c1": lambda: driver_fixture("ref_rkf78_c1", S.make_c1(n_tp=64), S.RUNGE_KUTTA_FEHLBERG78,
+                                           1e-10, 40, S.make_c1(n_tp=64).h_next),
+    "ref_rkf78_c2": lambda: driver_fixture("ref_rkf78_c2", S.make_c2(n_pl=128), S.RUNGE_KUTTA_FEHLBERG78,
+                                           1e-10, 30, S.make_c2(n_pl=128).h_next),
+    "ref_rk4_c5": lambda: driver_fixture("ref_rk4_c5", S.make_c5(n_tp=64), S.RUNGE_KUTTA4, 1e-10, 30, 1.0),
+    "ref_dp_c2": lambda: driver_fixture("ref_dp_c2", S.make_c2(n_pl=128), S.DORMAND_PRINCE, 1e-10, 30, 5.0),
+}
+
+
+def main():
+    if not cpu_ref.available("ref"):
+        raise SystemExit("oracle/_ref/libsolref.so is not built (make -C oracle ref)")
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            FIXTURES[name]()
+            print("wrote", name)
+        return
+    for name in FIXTURES:
+        subprocess.run([sys.executable, os.path.abspath(__file__), name], check=True,
+                       stdout=subprocess.DEVNULL)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,395 @@
+"""Pins the oracle (oracle/cpu_ref.c) before it is trusted as the checker.
+
+1. Golden vectors written by the compiled reference (tests/golden/*.npz):
+   the oracle must reproduce them bit for bit (same loop and evaluation
+   order, no FMA contraction).
+2. The reference's own unit-test vectors (src/Solaris.NBody.Cuda.Test/
+   unit_test.cpp, Test/Test.cpp) for the gas and orbit helpers, constants
+   and integrals, and for the seeded input generator.
+3. What the mathematics fixes, independent of the code's evaluation order:
+   heliocentric equations of motion in 40-digit arithmetic, momentum
+   conservation of the barycentric direct sum, and the convergence order of
+   each integrator on a Kepler orbit (a dropped or mistyped tableau entry
+   breaks the order).
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from oracle import cpu_ref
+from oracle.cpu_ref import Oracle
+from solaris_amd import _abi, scenarios as S
+
+import fixtures
+
+
+# ---- 1. reference golden vectors -------------------------------------------
+@pytest.mark.parametrize("name", fixtures.COMPUTE)
+def test_oracle_reproduces_reference_compute(name):
+    scn, out = fixtures.load(name)
+    o = Oracle(scn, barycentric=bool(out["barycentric"]))
+    dydt = o.compute(float(out["t"]), scn.y0)
+    # equal_nan: transition-regime bodies with Cd = 0 give log10(0) in the
+    # reference's drag (Solaris/Acceleration.cpp:389-394), i.e. NaN
+    assert np.array_equal(dydt, out["dydt"], equal_nan=True), np.nanmax(np.abs(dydt - out["dydt"]))
+    rm3, nn, dist, mig = o.side_outputs()
+    assert np.array_equal(rm3, out["rm3"])
+    assert np.array_equal(nn, out["index_nn"])
+    assert np.array_equal(dist, out["distance_nn"])
+    assert np.array_equal(mig, out["mig_type_after"])
+
+
+@pytest.mark.parametrize("name", fixtures.DRIVERS)
+def test_oracle_reproduces_reference_driver(name):
+    scn, out = fixtures.load(name)
+    o = Oracle(scn)
+    o.set_state(scn.y0)
+    tl = _abi.timeline(0.0, float(out["h0"]), 1e12)
+    hd, snaps = [], []
+    for k in range(len(out["hdid"])):
+        r, _ = o.driver(int(out["integrator"]), float(out["epsilon"]), tl)
+        assert r == 0
+        hd.append(tl.hDid)
+        if (k + 1) % 10 == 0:
+            snaps.append(o.get_array(0))
+    assert np.array_equal(np.array(hd), out["hdid"])
+    assert np.array_equal(np.array(snaps), out["snapshots"])
+    for key, which in (("y0_final", 0), ("y_final", 1), ("accel_final", 2), ("yscale_final", 3),
+                       ("error_final", 4)):
+        assert np.array_equal(o.get_array(which), out[key]), key
+    assert np.array_equal(o.integrals(), out["integrals_final"])
+    rm3, nn, dist, mig = o.side_outputs()
+    assert np.array_equal(rm3, out["rm3"]) and np.array_equal(nn, out["index_nn"])
+    assert np.array_equal(mig, out["mig_type_after"])
+
+
+def test_drag_fixture_reaches_every_regime():
+    scn, out = fixtures.load("ref_drag_regimes")
+    g = scn.gas
+    lo = scn.n_massive()
+    r = np.linalg.norm(scn.y0[lo:, :3], axis=1)
+    lam = g["mean_free_path"][0] * r ** g["mean_free_path"][1]
+    R = scn.radius[lo:]
+    assert (R <= 0.1 * lam).sum() >= 5          # Epstein
+    assert (R >= 10.0 * lam).sum() >= 5         # Stokes
+    assert ((R > 0.1 * lam) & (R < 10.0 * lam)).sum() >= 5   # transition
+
+
+@pytest.mark.skipif(not cpu_ref.available("ref"), reason="reference build not present")
+def test_default_gas_matches_reference_constructor():
+    fo, fr = cpu_ref.helper("cpu"), cpu_ref.helper("ref")
+    a, b = _abi.GasComponent(), _abi.GasComponent()
+    fo["default_gas"](a)
+    fr["default_gas"](b)
+    assert _abi.gas_to_dict(a) == _abi.gas_to_dict(b)
+    py = S.default_gas()
+    da = _abi.gas_to_dict(a)
+    for k in ("alpha", "mean_molecular_weight", "particle_diameter", "inner_edge"):
+        assert py[k] == da[k]
+    for k in ("eta", "tau", "scale_height", "density", "mean_free_path"):
+        assert tuple(py[k]) == da[k], k
+
+
+# ---- 2. the reference's own unit-test vectors ------------------------------
+def D(v):
+    return np.ascontiguousarray(v, dtype=np.float64).ctypes.data_as(cpu_ref.D)
+
+
+def vec3(f, *args):
+    out = np.zeros(3)
+    f(*args, D(out) if False else out.ctypes.data_as(cpu_ref.D))
+    return out
+
+
+def circ(mu, r):
+    h = cpu_ref.helper("cpu")
+    out = np.zeros(3)
+    rr = np.array(r, dtype=np.float64)
+    h["circular_velocity"](mu, rr.ctypes.data_as(cpu_ref.D), out.ctypes.data_as(cpu_ref.D))
+    return out
+
+
+def test_circular_velocity_vectors():
+    # src/Solaris.NBody.Cuda.Test/unit_test.cpp:558-604
+    assert list(circ(1.0, [1.0, 0.0, 0.0])) == [0.0, 1.0, 0.0]
+    assert list(circ(1.0, [0.0, 1.0, 0.0])) == [-1.0, 0.0, 0.0]
+    assert list(circ(1.0, [-1.0, 0.0, 0.0])) == [0.0, -1.0, 0.0]
+    assert list(circ(1.0, [0.0, -1.0, 0.0])) == [1.0, 0.0, 0.0]
+    v = circ(0.001, [5.0, 0.5, 0.0])
+    assert abs(-0.0014036989255830 - v[0]) <= 1e-15
+    assert abs(0.01403698925583099 - v[1]) <= 1e-15
+    assert v[2] == 0.0
+
+
+def gas(**kw):
+    g = S.default_gas()
+    g.update(kw)
+    return _abi.gas_component(g)
+
+
+def test_gas_velocity_vector():
+    # unit_test.cpp:615-638: eta = (1e-3, 0.5), mu = 1 at (1, 0, 0)
+    h = cpu_ref.helper("cpu")
+    g = gas(eta=(1.0e-3, 0.5))
+    out = np.zeros(3)
+    r = np.array([1.0, 0.0, 0.0])
+    h["gas_velocity"](g, 1.0, r.ctypes.data_as(cpu_ref.D), out.ctypes.data_as(cpu_ref.D))
+    assert out[0] == 0.0 and out[2] == 0.0
+    assert out[1] == 0.99899949949937412368543414284205
+
+
+@pytest.mark.parametrize("r, expected, tol", [
+    # unit_test.cpp:640-699 (rho = 1e-10 r^-3, H = 0.05 r^1.5, inner edge 0.1 AU)
+    ((0.1, 0.0, 0.0), 1.0e-7, 1e-15),
+    ((0.05, 0.0, 0.0), 6.25e-9, 1e-15),
+    ((1.0, 0.0, 0.0), 1.0e-10, 0.0),
+    ((0.0, 1.0, 0.0), 1.0e-10, 0.0),
+    ((1.0, 0.0, 5.0e-2), 3.6787944117144232159552377016146e-11, 1e-16),
+])
+def test_gas_density_vectors(r, expected, tol):
+    h = cpu_ref.helper("cpu")
+    g = gas(density=(1.0e-10, -3.0), scale_height=(5.0e-2, 1.5), inner_edge=0.1)
+    rr = np.array(r, dtype=np.float64)
+    v = h["gas_density_at"](g, rr.ctypes.data_as(cpu_ref.D))
+    assert abs(v - expected) <= tol
+
+
+def test_temperature_and_thermal_speed_vectors():
+    # Test/Test.cpp:480-528 (default GasComponent: H = 0.02 r^1.25)
+    h = cpu_ref.helper("cpu")
+    g = gas()
+    assert abs(h["temperature_cmu"](g, 1.0, 1.0) - 98.903471085889933) <= 1.0
+    assert abs(h["temperature_cmu"](g, 1.0, 0.1) - 312.67170117983335) <= 1.0
+    mps_to_aupd = (1.0 / 1.495978707e11) / (1.0 / 86400.0)
+    assert abs(h["mean_thermal_speed_cmu"](g, 1.0, 1.0) - 950.7244052398592 * mps_to_aupd) <= 1e-5
+    # the argument swap inside MeanThermalSpeed_CMU (Solaris/GasComponent.cpp:241):
+    # v_th(mC, r) = C * sqrt(T(mC = r, r = mC))
+    T_sw = h["temperature_cmu"](g, 2.0, 1.0)
+    v = h["mean_thermal_speed_cmu"](g, 1.0, 2.0)
+    assert v == pytest.approx(h["mean_thermal_speed_cmu"](g, 1.0, 1.0) * math.sqrt(T_sw / h["temperature_cmu"](g, 1.0, 1.0)),
+                              rel=1e-15)
+
+
+def test_constants():
+    h = cpu_ref.helper("cpu")
+    # Test/Test.cpp:74-82
+    assert abs(h["constant"](b"GramPerCm3ToSolarPerAu3") - 1.6831291259964250163336161614994e6) <= 1e-10
+    assert abs(S.GCM3_TO_SOLAR_AU3 - 1.6831291259964250163336161614994e6) <= 1e-10
+    assert h["constant"](b"Gauss2") == S.GAUSS2 == 2.959122082855911025e-4
+    # unit_test.cpp:905-927: Earth's period from k, 365.25635 d
+    mu = S.GAUSS2 * (1.0 + 1.0 / 3.3294605e5)
+    assert abs(2.0 * S.PI * math.sqrt(1.0 / mu) - 365.25635) <= 1e-5
+
+
+EPHEM_R = (-1.824214307389804, +0.9081741252311457, -0.3136484471795240)
+EPHEM_V = (-4.959970348405931E-03, -9.644216216654788E-03, +4.150430491295794E-03)
+
+
+def test_orbital_a_e_vector():
+    # unit_test.cpp:867-881: (a, e) of the ephemeris state with mu = k^2
+    h = cpu_ref.helper("cpu")
+    rv = np.array(EPHEM_R + EPHEM_V)
+    a, e = np.zeros(1), np.zeros(1)
+    assert h["orbital_a_e"](S.GAUSS2, rv.ctypes.data_as(cpu_ref.D), a.ctypes.data_as(cpu_ref.D),
+                            e.ctypes.data_as(cpu_ref.D)) == 0
+    assert abs(a[0] - 1.944132) <= 1e-6 and abs(e[0] - 0.0737774) <= 1e-7
+    # unbound orbit: returns 1 and leaves (a, e) untouched (Solaris/Ephemeris.cpp:16-21)
+    rv2 = rv.copy(); rv2[3:] *= 10.0
+    a[0] = e[0] = -7.0
+    assert h["orbital_a_e"](S.GAUSS2, rv2.ctypes.data_as(cpu_ref.D), a.ctypes.data_as(cpu_ref.D),
+                            e.ctypes.data_as(cpu_ref.D)) == 1
+    assert a[0] == -7.0 and e[0] == -7.0
+
+
+def test_generator_kepler_and_elements():
+    # unit_test.cpp:764-817 (e = 0.5) and :819-865 (elements -> state, 1e-14)
+    deg = 0.0174532925199432957692
+    M = np.array([27.0, 125.0, 205.0, 300.0, 350.0]) * deg
+    E = S.kepler_solve(M, np.full(5, 0.5)) / deg
+    np.testing.assert_allclose(E, [48.43417991487915, 142.4568442405949, 196.7457972979956,
+                                   271.36018243209764, 340.38113495327434], rtol=0, atol=1e-10)
+    y = S.elements_to_cartesian(S.GAUSS2, np.array([1.944132]), np.array([0.0737774]),
+                                np.array([22.50914 * deg]), np.array([124.04003 * deg]),
+                                np.array([175.33712 * deg]), np.array([217.31447 * deg]))[0]
+    np.testing.assert_allclose(y[:3], EPHEM_R, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(y[3:], EPHEM_V, rtol=0, atol=1e-14)
+
+
+def triangle(vel=True):
+    s = S._assemble("triangle", S._counts(central_body=1, giant_planet=2),
+                    [S._star(1.0 / 3.0),
+                     dict(mass=np.array([1.0 / 3.0, 1.0 / 3.0]), type=S.GIANT, y0=np.zeros((2, 6)))])
+    s.y0[1] = (1.0, 0.0, 0.0, 0.0, 1.0 if vel else 0.0, 0.0)
+    q = math.sqrt(3.0) / 2.0
+    s.y0[2] = (0.5, q, 0.0, -q if vel else 0.0, 0.5 if vel else 0.0, 0.0)
+    return s
+
+
+def test_integrals_triangle_vectors():
+    # Test/Test.cpp:123-419: equal-mass equilateral triangle, side 1
+    m = 1.0 / 3.0
+    o = Oracle(triangle(vel=False))
+    o.set_state(triangle(vel=False).y0)
+    out = o.integrals()
+    assert abs(out[0] - 1.0) <= 1e-10
+    np.testing.assert_allclose(out[1:7], [0.5, 1.0 / (2.0 * math.sqrt(3.0)), 0, 0, 0, 0], atol=1e-14)
+    s = triangle(vel=True)
+    o = Oracle(s)
+    o.set_state(s.y0)
+    out = o.integrals()
+    np.testing.assert_allclose(out[9:12], [0.0, 0.0, 2 * m], atol=1e-14)
+    assert abs(out[14] - S.GAUSS2 / 3.0) <= 1e-14      # potential (positive sum)
+    assert abs(out[13] - m) <= 1e-14                   # kinetic
+    assert abs(out[15] - 1.0 / 3.0 * (1.0 - S.GAUSS2)) <= 1e-14
+
+
+# ---- 3. what the mathematics fixes ------------------------------------------
+def heliocentric_mp(scn, y, digits=40):
+    """Heliocentric equations of motion (e.g. Murray & Dermott, Solar System
+    Dynamics, eq. 6.8) in 40-digit arithmetic, with the reference's
+    interaction sets: bodies of type <= ProtoPlanet feel massive bodies and
+    super-planetesimals, all others feel massive bodies only."""
+    mp.mp.dps = digits
+    k2 = mp.mpf(S.GAUSS2)
+    n = scn.n
+    nm = scn.n_massive()
+    end_a = nm + scn.counts["super_planetesimal"]
+    Y = [[mp.mpf(float(v)) for v in row] for row in y]
+    m = [mp.mpf(float(v)) for v in scn.mass]
+    out = np.zeros((n, 3))
+    for i in range(1, n):
+        ri = Y[i][:3]
+        r3 = mp.sqrt(sum(c * c for c in ri)) ** 3
+        a = [-k2 * (m[0] + m[i]) * c / r3 for c in ri]
+        end = end_a if scn.type[i] <= S.PROTO else nm
+        for j in range(1, end):
+            if j == i:
+                continue
+            rj = Y[j][:3]
+            d = [rj[c] - ri[c] for c in range(3)]
+            d3 = mp.sqrt(sum(c * c for c in d)) ** 3
+            rj3 = mp.sqrt(sum(c * c for c in rj)) ** 3
+            for c in range(3):
+                a[c] += k2 * m[j] * (d[c] / d3 - rj[c] / rj3)
+        out[i] = [float(v) for v in a]
+    return out
+
+
+def mixed_types_system():
+    rng = S.SplitMix64(77)
+    groups = [S._star()]
+    spec = [(S.GIANT, 2, 1e-3), (S.ROCKY, 1, 3e-6), (S.PROTO, 2, 1e-6), (S.SUPERPL, 2, 1e-8),
+            (S.PL, 3, 1e-10), (S.TP, 2, 0.0)]
+    counts = S._counts(central_body=1, giant_planet=2, rocky_planet=1, proto_planet=2,
+                       super_planetesimal=2, planetesimal=3, test_particle=2)
+    for typ, k, mm in spec:
+        a = rng.uniform(k, 0.8, 6.0)
+        y = S._orbits(rng, k, 1.0, mm, a, rng.uniform(k, 0, 0.2), rng.uniform(k, 0, 0.3))
+        groups.append(dict(mass=np.full(k, mm), type=typ, y0=y))
+    return S._assemble("mixed", counts, groups)
+
+
+def test_oracle_gravity_against_40_digit_equations_of_motion():
+    scn = mixed_types_system()
+    o = Oracle(scn)
+    a = o.gravity(scn.y0)[:, 3:]
+    ref = heliocentric_mp(scn, scn.y0)
+    err = np.linalg.norm(a - ref, axis=1)[1:] / np.linalg.norm(ref, axis=1)[1:]
+    assert err.max() <= 1e-14, err.max()
+    # a dropped indirect term or a wrong interaction set is far outside this
+
+
+def test_oracle_star_plus_test_particle_is_pure_kepler():
+    scn, out = fixtures.load("ref_tp_only")
+    mp.mp.dps = 40
+    for i in (1, 7, 23):
+        r = [mp.mpf(float(v)) for v in scn.y0[i, :3]]
+        r3 = mp.sqrt(sum(c * c for c in r)) ** 3
+        exact = [float(-mp.mpf(S.GAUSS2) * c / r3) for c in r]
+        # sqrt, divide and two products: a few ulps
+        np.testing.assert_allclose(out["dydt"][i, 3:], exact, rtol=1e-15)
+
+
+def test_barycentric_direct_sum_conserves_momentum():
+    scn = S._assemble("massive", S._counts(central_body=1, giant_planet=4),
+                      [S._star(), dict(mass=np.array([1e-3, 3e-4, 5e-5, 4e-5]), type=S.GIANT,
+                                       y0=S._orbits(S.SplitMix64(5), 4, 1.0, np.array([1e-3, 3e-4, 5e-5, 4e-5]),
+                                                    np.array([5.2, 9.5, 19.2, 30.1]), np.full(4, 0.05),
+                                                    np.full(4, 0.02)))])
+    o = Oracle(scn, barycentric=True)
+    a = o.compute(0.0, scn.y0)[:, 3:]
+    p = (scn.mass[:, None] * a).sum(0)
+    assert np.abs(p).max() <= 1e-19 * np.abs(scn.mass[:, None] * a).max() * 1e3
+
+
+def kepler_exact(mu, y0, t):
+    """Two-body propagation through the universal Kepler solution (elements
+    from the state, mean anomaly advanced, state rebuilt)."""
+    r = y0[:3]; v = y0[3:]
+    rn = np.linalg.norm(r)
+    h = np.cross(r, v)
+    evec = np.cross(v, h) / mu - r / rn
+    e = np.linalg.norm(evec)
+    a = 1.0 / (2.0 / rn - v @ v / mu)
+    n = math.sqrt(mu / a ** 3)
+    cosE0 = (1.0 - rn / a) / e
+    sinE0 = (r @ v) / (e * math.sqrt(mu * a))
+    E0 = math.atan2(sinE0, cosE0)
+    M = E0 - e * math.sin(E0) + n * t
+    E = S.kepler_solve(np.array([M % (2 * S.PI)]), np.array([e]))[0]
+    # f and g functions
+    dE = E - E0
+    dE = (dE + S.PI) % (2 * S.PI) - S.PI
+    f = 1.0 - a / rn * (1.0 - math.cos(dE))
+    g = t - (dE - math.sin(dE)) / n
+    rnew = f * r + g * v
+    rr = np.linalg.norm(rnew)
+    fd = -math.sqrt(mu * a) / (rr * rn) * math.sin(dE)
+    gd = 1.0 - a / rr * (1.0 - math.cos(dE))
+    return np.concatenate([rnew, fd * r + gd * v])
+
+
+@pytest.mark.parametrize("integrator, order", [(S.RUNGE_KUTTA4, 4), (S.RUNGE_KUTTA_FEHLBERG78, 7),
+                                               (S.DORMAND_PRINCE, 6)])
+def test_integrator_convergence_order(integrator, order):
+    """Local error of one step on an e = 0.3 Kepler orbit shrinks as
+    h^(order+1): the published tableaux (classical RK4; Fehlberg 7(8), NASA
+    TR R-381, advanced with its 7th-order weights; Dormand & Prince RKN
+    7(6), Celest. Mech. 18, 1978, whose position update in Step2 -
+    DormandPrince.cpp:475-476 - converges as h^7 per step) fix it.  A
+    mistyped or dropped coefficient breaks consistency and the rate."""
+    rng = S.SplitMix64(11)
+    y = S._orbits(rng, 1, 1.0, 0.0, np.array([1.0]), np.array([0.3]), np.array([0.1]), massless=True)
+    scn = S._assemble("kep", S._counts(central_body=1, test_particle=1),
+                      [S._star(), dict(mass=np.zeros(1), type=S.TP, y0=y)])
+    mu = S.GAUSS2
+    errs = []
+    hs = [16.0, 8.0, 4.0] if order == 4 else [20.0, 10.0, 5.0]
+    for h in hs:
+        o = Oracle(scn)
+        o.set_state(scn.y0)
+        o.set_accel(o.compute(0.0, scn.y0))
+        o.step(integrator, 0.0, h)
+        y1 = o.get_array(1)[1]
+        errs.append(np.linalg.norm(y1[:3] - kepler_exact(mu, scn.y0[1], h)[:3]))
+    rates = [math.log2(errs[k] / errs[k + 1]) for k in range(len(errs) - 1)]
+    for r in rates:
+        assert abs(r - (order + 1)) < 0.6, (rates, errs)
+
+
+def test_openmp_build_matches_single_thread():
+    scn = S.make_c3(n=512, softening=0.0)
+    a = Oracle(scn).compute(0.0, scn.y0)
+    b = Oracle(scn, lib="omp").compute(0.0, scn.y0)
+    assert np.array_equal(a, b)
+
+
+def test_interactions_per_evaluation_formula():
+    assert S.make_c1().interactions_per_evaluation() == 1540.0
+    c3 = S.make_c3(n=256)
+    assert c3.interactions_per_evaluation() == 256.0 * 256.0
+    assert Oracle(c3).interactions_per_evaluation() == 256.0 * 256.0
+    c2 = S.make_c2(n_pl=100, n_proto=4)
+    assert Oracle(c2).interactions_per_evaluation() == c2.interactions_per_evaluation() == 4 * 4 + 100 * 5
