@@ -61,16 +61,34 @@ def build_hip(verbose: bool = False) -> str:
 
 
 def build_dropin(verbose: bool = False) -> str:
-    srcs = sorted(os.path.join(DROPIN, f) for f in os.listdir(DROPIN) if f.endswith(".cpp")) \
-        if os.path.isdir(DROPIN) else []
-    if not srcs:
-        return ""
+    """The drop-in classes (dropin/*.cpp) with the standalone definitions of
+    the plain data classes (dropin/standalone/*.cpp, dropin/include/*.h),
+    and the C++ driver program tests/cpp/dropin_driver."""
+    srcs = sorted(os.path.join(DROPIN, f) for f in os.listdir(DROPIN) if f.endswith(".cpp"))
+    std = os.path.join(DROPIN, "standalone")
+    srcs += sorted(os.path.join(std, f) for f in os.listdir(std) if f.endswith(".cpp"))
+    inc = [f"-I{os.path.join(DROPIN, 'include')}", f"-I{DROPIN}", f"-I{os.path.join(ROOT, 'include')}"]
     tmp = DROPIN_LIB + ".tmp"
-    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{DROPIN}",
-          f"-I{os.path.join(ROOT, 'include')}", "-o", tmp, *srcs,
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", *inc, "-o", tmp, *srcs,
           f"-L{HERE}", "-lsolaris_amd", "-Wl,-rpath,$ORIGIN"], not verbose)
     os.replace(tmp, DROPIN_LIB)
+    drv_src = os.path.join(ROOT, "tests", "cpp", "dropin_driver.cpp")
+    if os.path.exists(drv_src):
+        exe = os.path.join(ROOT, "tests", "cpp", "dropin_driver")
+        _run(["g++", "-O2", "-std=c++17", "-Wall", *inc, "-o", exe + ".tmp", drv_src,
+              f"-L{HERE}", "-lsolaris_dropin", "-lsolaris_amd", f"-Wl,-rpath,{HERE}"], not verbose)
+        os.replace(exe + ".tmp", exe)
     return DROPIN_LIB
+
+
+def check_dropin_against(reference_include: str) -> None:
+    """Compiles the drop-in set against another tree's headers (the
+    reference's Solaris/ directory) to show the classes keep its
+    interface."""
+    for f in ("Acceleration", "RungeKuttaFehlberg78", "RungeKutta4", "DormandPrince", "dropin_bridge"):
+        _run(["g++", "-std=c++11", "-fsyntax-only", "-fpermissive", "-w", "-include", "cstring",
+              "-include", "cmath", "-include", "stdlib.h", f"-I{reference_include}", f"-I{DROPIN}",
+              f"-I{os.path.join(ROOT, 'include')}", os.path.join(DROPIN, f + ".cpp")])
 
 
 def build_all(verbose: bool = False):
