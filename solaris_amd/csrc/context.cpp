@@ -124,6 +124,7 @@ struct sol_context {
     double *y0 = nullptr, *y = nullptr, *acc0 = nullptr, *yscale = nullptr, *error = nullptr;
     double *fst = nullptr, *cin = nullptr, *cout = nullptr, *aos = nullptr;
     double4 *J = nullptr;
+    float4 *J32 = nullptr;     // allocated when the mixed form is selected
     double *part = nullptr, *nnr2 = nullptr;
     int32_t *nnidx = nullptr;
     double *ind = nullptr;
@@ -177,6 +178,8 @@ int dalloc(sol_context *c, T **p, size_t count)
 }
 
 int n_pl_spl(const sol_nbodies &nb) { return nb.super_planetesimal + nb.planetesimal; }
+
+bool mixed_on(const sol_context *c) { return c->precision == SOL_PRECISION_MIXED && c->J32 != nullptr; }
 
 // Source range ends of the two astrocentric sink classes
 // (Solaris/Acceleration.cpp:285-289).
@@ -338,6 +341,8 @@ EpiArgs base_args(sol_context *c)
     a.drag_eval = c->eval_drag;
     a.mig_on = gas_live ? 1 : 0;
     a.J = c->J;
+    a.J32 = mixed_on(c) ? c->J32 : nullptr;
+    a.mixed = mixed_on(c) ? 1 : 0;
     a.mass = c->mass; a.radius = c->radius; a.density = c->density; a.cd = c->cd; a.gs = c->gs; a.ge = c->ge;
     a.migt = c->migt; a.migc = c->migc; a.dragc = c->dragc;
     a.gas = c->gasdev;
@@ -370,7 +375,8 @@ int evaluate_at(sol_context *c, const double *st, double t, double h, double *ou
     if (r) return r;
     int ea, eb;
     source_ends(c, ea, eb);
-    launch_prep(c->stream, c->n, c->np, st, c->gm, c->J, ind_buf(c, 0), c->barycentric, ea, eb, nullptr);
+    launch_prep(c->stream, c->n, c->np, st, c->gm, c->J, mixed_on(c) ? c->J32 : nullptr, ind_buf(c, 0),
+                c->barycentric, ea, eb);
     const bool gas_live = c->has_gas && !c->gas_dead;
     if (gas_live && (c->eval_mig1 || c->eval_mig2)) {
         const int do1 = c->eval_mig1 && c->nb.proto_planet > 0;
@@ -382,7 +388,7 @@ int evaluate_at(sol_context *c, const double *st, double t, double h, double *ou
     {
         ProfScope ps(c, 0);
         launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
-                     c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+                     mixed_on(c), c->softening * c->softening, c->J32);
     }
     EpiArgs a = base_args(c);
     a.kind = kind;
@@ -439,7 +445,7 @@ int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double
         {
             ProfScope ps(c, 0);
             launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
-                         c->precision == SOL_PRECISION_MIXED, c->softening * c->softening, nullptr);
+                         mixed_on(c), c->softening * c->softening, c->J32);
         }
         EpiArgs e = a;
         e.stage = s;
@@ -717,8 +723,11 @@ int sol_set_precision(sol_context *c, int32_t precision, double softening)
     if (precision != SOL_PRECISION_FP64 && precision != SOL_PRECISION_MIXED)
         return c->fail(SOL_ERR_ARGUMENT, "unknown precision %d", precision);
     if (!(softening >= 0.0)) return c->fail(SOL_ERR_ARGUMENT, "softening must be >= 0");
-    if (precision == SOL_PRECISION_MIXED)
-        return c->fail(SOL_ERR_UNSUPPORTED, "mixed precision is not available in this build");
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (precision == SOL_PRECISION_MIXED && !c->J32) {
+        int r = dalloc(c, &c->J32, (size_t)c->np);
+        if (r) return r;
+    }
     c->precision = precision;
     c->softening = softening;
     return SOL_OK;

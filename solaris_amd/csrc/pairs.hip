@@ -138,16 +138,133 @@ void launch_fp64(hipStream_t st, const PairPlan &plan, const double4 *J, double 
                        plan, J, part, nnr2, nnidx, eps2, 1.5, 1.875);
 }
 
+// Mixed precision: FP32 pair terms (differences, |d|^-3 from v_rsq_f32 and
+// the products), summed in FP32 over one chunk of at most `chunk` sources
+// and folded into FP64 accumulators per chunk, so that the accumulation
+// error grows with the chunk length, not with N.  J32 holds the FP32
+// copies (x, y, z, k^2 m) of the same positions.  The nearest neighbour is
+// chosen on the FP32 squared distance; its distance is recomputed in FP64
+// by the epilogue.
+template <bool NN, bool BC, bool SOFT>
+__global__ __launch_bounds__(256) void k_pairs_mixed(PairPlan plan, const float4 *__restrict__ J32,
+                                                      double *__restrict__ part, double *__restrict__ nnr2,
+                                                      int32_t *__restrict__ nnidx, float eps2)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (w >= plan.nwaves) return;
+    const int64_t lo = ((int64_t)w * plan.total_items) / plan.nwaves;
+    const int64_t hi = ((int64_t)(w + 1) * plan.total_items) / plan.nwaves;
+
+    int slot = 0, cur = -1;
+    int sb = 0, se = 0, i = 0;
+    float xi = 0.f, yi = 0.f, zi = 0.f;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    float best = FLT_MAX;
+    int bidx = -1;
+
+    auto flush = [&](int k) {
+        const int64_t base = ((int64_t)w * plan.slots_per_wave + k) * (3 * kWave);
+        part[base + lane] = ax;
+        part[base + kWave + lane] = ay;
+        part[base + 2 * kWave + lane] = az;
+        if (NN) {
+            const int64_t nb = ((int64_t)w * plan.slots_per_wave + k) * kWave;
+            nnr2[nb + lane] = (double)best;
+            nnidx[nb + lane] = bidx;
+        }
+    };
+
+    for (int64_t it = lo; it < hi; ++it) {
+        int s, t, c;
+        decode_item(plan, it, s, t, c);
+        const PairSeg &sg = plan.seg[s];
+        const int g = sg.tile_begin + t;
+        if (g != cur) {
+            if (cur >= 0) { flush(slot); ++slot; }
+            cur = g;
+            ax = ay = az = 0.0;
+            best = FLT_MAX; bidx = -1;
+            sb = sg.sink_begin + t * kWave;
+            se = min(sb + kWave, sg.sink_end);
+            i = min(sb + lane, se - 1);
+            const float4 me = J32[i];
+            xi = me.x; yi = me.y; zi = me.z;
+        }
+        const int jb = sg.src_begin + c * plan.chunk;
+        const int je = min(jb + plan.chunk, sg.src_end);
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        if (jb >= se || sb >= je) {
+            // no self pair in this chunk
+#pragma unroll 4
+            for (int j = jb; j < je; ++j) {
+                const float4 q = J32[j];
+                const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const float y0 = __builtin_amdgcn_rsqf(SOFT ? r2 + eps2 : r2);
+                const float f = (q.w * y0) * (y0 * y0);
+                fx = __builtin_fmaf(f, dx, fx);
+                fy = __builtin_fmaf(f, dy, fy);
+                fz = __builtin_fmaf(f, dz, fz);
+                if (NN) {
+                    if (BC ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
+                }
+            }
+        } else {
+            for (int j = jb; j < je; ++j) {
+                const float4 q = J32[j];
+                const bool self = (j == i);
+                const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const float rs = self ? 1.0f : (SOFT ? r2 + eps2 : r2);
+                const float y0 = __builtin_amdgcn_rsqf(rs);
+                const float m = self ? 0.0f : q.w;
+                const float f = (m * y0) * (y0 * y0);
+                fx = __builtin_fmaf(f, dx, fx);
+                fy = __builtin_fmaf(f, dy, fy);
+                fz = __builtin_fmaf(f, dz, fz);
+                if (NN && !self) {
+                    if (BC ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
+                }
+            }
+        }
+        ax += (double)fx;
+        ay += (double)fy;
+        az += (double)fz;
+    }
+    if (cur >= 0) flush(slot);
+}
+
+template <bool NN, bool BC, bool SOFT>
+void launch_mixed(hipStream_t st, const PairPlan &plan, const float4 *J32, double *part,
+                  double *nnr2, int32_t *nnidx, float eps2)
+{
+    const int blocks = (plan.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((k_pairs_mixed<NN, BC, SOFT>), dim3(blocks), dim3(kBlock), 0, st,
+                       plan, J32, part, nnr2, nnidx, eps2);
+}
+
 }  // namespace
 
 void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
                   double *nnr2, int32_t *nnidx, bool nn, bool bc, bool mixed, double eps2,
-                  const double *jlo)
+                  const float4 *J32)
 {
-    (void)jlo;
-    (void)mixed;
     if (plan.total_items == 0) return;
     const bool soft = eps2 != 0.0;
+    if (mixed) {
+        const float e2 = (float)eps2;
+#define SOL_MIXED(NN, BC, SOFT) launch_mixed<NN, BC, SOFT>(st, plan, J32, part, nnr2, nnidx, e2)
+        if (nn) {
+            if (bc) { if (soft) SOL_MIXED(true, true, true); else SOL_MIXED(true, true, false); }
+            else    { if (soft) SOL_MIXED(true, false, true); else SOL_MIXED(true, false, false); }
+        } else {
+            if (bc) { if (soft) SOL_MIXED(false, true, true); else SOL_MIXED(false, true, false); }
+            else    { if (soft) SOL_MIXED(false, false, true); else SOL_MIXED(false, false, false); }
+        }
+#undef SOL_MIXED
+        return;
+    }
 #define SOL_PAIRS(NN, BC, SOFT) launch_fp64<NN, BC, SOFT>(st, plan, J, part, nnr2, nnidx, eps2)
     if (nn) {
         if (bc) { if (soft) SOL_PAIRS(true, true, true); else SOL_PAIRS(true, true, false); }

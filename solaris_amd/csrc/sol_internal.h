@@ -72,6 +72,8 @@ struct EpiArgs {
     int32_t mig_on, kind, integ, stage;
     int32_t last, want_nn, drag_eval, pad1;
     const double4 *J;         // stage positions (and k^2 m); rewritten in place for the next stage
+    float4 *J32;              // FP32 copy of J for the mixed-precision pair sum (or null)
+    int32_t mixed, pad2;
     const double *vel;        // stage velocities (3 x NP)
     const double *mass, *radius, *density, *cd, *gs, *ge;
     const int32_t *migt;      // BodyData::migType on the device
@@ -95,10 +97,10 @@ struct EpiArgs {
 // Host-side launchers (pairs.hip / stages.hip).
 void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
                   double *nnr2, int32_t *nnidx, bool nn, bool barycentric, bool mixed,
-                  double eps2, const double *jlo);
+                  double eps2, const float4 *J32);
 void launch_prep(hipStream_t st, int n, int np, const double *state, const double *gm,
-                 double4 *J, double *ind_out, int barycentric, int src_end_a, int src_end_b,
-                 double *rm3_side);
+                 double4 *J, float4 *J32, double *ind_out, int barycentric, int src_end_a,
+                 int src_end_b);
 void launch_migration(hipStream_t st, int n_central, int n_giant, int n_massive, int np,
                       const double *state, const double *mass, const double *stop,
                       int32_t *migt, double *migc, const GasDev &gas, int barycentric,

@@ -261,16 +261,18 @@ __device__ __forceinline__ void indirect_contrib(int i, int n, double x, double 
 
 __global__ __launch_bounds__(256) void k_prep(int n, int np, const double *__restrict__ st,
                                                const double *__restrict__ gm, double4 *__restrict__ J,
-                                               double *__restrict__ ind_out, int bc, int ea, int eb)
+                                               float4 *__restrict__ J32, double *__restrict__ ind_out,
+                                               int bc, int ea, int eb)
 {
     __shared__ double lds[kBlock / kWave][6];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
     if (i < n) {
         x = st[i]; y = st[np + i]; z = st[2 * np + i]; m = gm[i];
+    }
+    if (i < np) {
         J[i] = make_double4(x, y, z, m);
-    } else if (i < np) {
-        J[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (J32) J32[i] = make_float4((float)x, (float)y, (float)z, (float)m);
     }
     if (bc) return;
     double v[6];
@@ -392,6 +394,7 @@ __device__ void emit_next(const EpiArgs &a, int i, const double y[6], double v[6
     const double gm = kGauss2 * a.mass[i];
     double4 *J = const_cast<double4 *>(a.J);
     J[i] = make_double4(y[0], y[1], y[2], gm);
+    if (a.J32) a.J32[i] = make_float4((float)y[0], (float)y[1], (float)y[2], (float)gm);
     a.fst_next[i] = y[3];
     a.fst_next[a.np + i] = y[4];
     a.fst_next[2 * a.np + i] = y[5];
@@ -476,7 +479,17 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
         if (a.last) {
             if (!a.barycentric) a.rm3_out[i] = rm3;
             a.nni_out[i] = (a.want_nn && bidx >= 0) ? bidx : -1;
-            a.nnd_out[i] = (a.want_nn && bidx >= 0) ? sqrt(best) : 0.0;
+            double dnn = 0.0;
+            if (a.want_nn && bidx >= 0) {
+                if (a.mixed) {   // the FP32 search picked the index; the distance is FP64
+                    const double4 q = a.J[bidx];
+                    const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+                    dnn = sqrt(dx * dx + dy * dy + dz * dz);
+                } else {
+                    dnn = sqrt(best);
+                }
+            }
+            a.nnd_out[i] = dnn;
         }
     }
 
@@ -640,10 +653,10 @@ inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
 }  // namespace
 
 void launch_prep(hipStream_t st, int n, int np, const double *state, const double *gm, double4 *J,
-                 double *ind_out, int bc, int ea, int eb, double *rm3_side)
+                 float4 *J32, double *ind_out, int bc, int ea, int eb)
 {
-    (void)rm3_side;
-    hipLaunchKernelGGL(k_prep, dim3(np / kBlock), dim3(kBlock), 0, st, n, np, state, gm, J, ind_out, bc, ea, eb);
+    hipLaunchKernelGGL(k_prep, dim3(np / kBlock), dim3(kBlock), 0, st, n, np, state, gm, J, J32, ind_out, bc,
+                       ea, eb);
 }
 
 void launch_migration(hipStream_t st, int n_central, int n_giant, int n_massive, int np, const double *state,
