@@ -81,6 +81,17 @@ def build_dropin(verbose: bool = False) -> str:
     return DROPIN_LIB
 
 
+def device_assembly(src: str) -> str:
+    """gfx950 assembly of one HIP source of this library, compiled with the
+    library's own flags (for inspection by the tests)."""
+    out = os.path.join(BUILD, os.path.basename(src) + ".s")
+    os.makedirs(BUILD, exist_ok=True)
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    _run([HIPCC, *HIP_FLAGS, *lang, "--cuda-device-only", "-S", os.path.join(CSRC, src), "-o", out])
+    with open(out) as f:
+        return f.read()
+
+
 def check_dropin_against(reference_include: str) -> None:
     """Compiles the drop-in set against another tree's headers (the
     reference's Solaris/ directory) to show the classes keep its

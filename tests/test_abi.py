@@ -37,3 +37,39 @@ def test_version_and_error_without_device():
     r = lib.sol_create(ctypes.byref(h), ctypes.byref(nb), 0, 0)
     assert r == 6  # SOL_ERR_ALLOCATION, BodyData::Allocate with total == 0
     assert b"allocation" in lib.sol_last_error(None)
+
+
+def test_device_code_has_no_packed_fp32_instructions():
+    """Every HIP source is compiled with packed FP32 disabled; the gfx950
+    assembly must contain no v_pk_{add,mul,fma}_f32."""
+    import re
+    from solaris_amd import build
+    for src in build.HIP_SOURCES:
+        asm = build.device_assembly(src)
+        assert "v_rsq" in asm or src == "context.cpp"
+        assert not re.search(r"v_pk_(add|mul|fma)_f32", asm), src
+
+
+def test_pair_kernel_inner_loop_instruction_count():
+    """The FP64 pair loop is 16 FP64 VALU operations plus one v_rsq_f64 per
+    pair, the mixed one 12 FP32 operations plus one v_rsq_f32 (unrolled by
+    four; SGPR-fed source records)."""
+    import collections
+    import re
+    from solaris_amd import build
+    asm = build.device_assembly("pairs.hip")
+    names = re.findall(r"^(_Z\S+):", asm, re.M)
+
+    def loop_valu(tag):
+        sym = [n for n in names if tag in n][0]
+        start = asm.index(sym + ":")
+        body = asm[start:asm.index("s_endpgm", start)]
+        blocks = re.split(r"\n\.LBB\S+:[^\n]*", body)
+        best = max(blocks, key=lambda b: len(re.findall(r"^\s+v_", b, re.M)))
+        c = collections.Counter(l.split()[0] for l in best.split("\n")
+                                if l.strip().startswith("v_"))
+        return c
+    f = loop_valu("k_pairs_fp64ILb0ELb0ELb0E")
+    assert sum(f.values()) == 4 * 17 + 1 and f["v_rsq_f64_e32"] == 4   # + one loop-carried move
+    m = loop_valu("k_pairs_mixedILb0ELb0ELb0E")
+    assert sum(m.values()) == 4 * 13 and m["v_rsq_f32_e32"] == 4
