@@ -10,11 +10,13 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "sol_internal.h"
+#include "resident.h"
 
 using namespace sol;
 
@@ -135,6 +137,10 @@ struct sol_context {
     double *h_scal = nullptr;   // pinned: [0] h, [1] t, [2] error max
     size_t part_slots = 0;
     std::vector<void *> allocs;
+    // workgroup-resident drivers (resident.hip) for small gas-free systems
+    ResSystem *res_d = nullptr, *res_h = nullptr;
+    bool resident_disabled = false;
+    size_t lds_limit = 65536;
     // profiling (sol_profile)
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -472,8 +478,73 @@ int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double
     return SOL_OK;
 }
 
+// Small systems without gas run whole Driver loops in one workgroup.
+bool use_resident(const sol_context *c)
+{
+    if (c->resident_disabled) return false;
+    if (c->has_gas && !c->gas_dead) return false;
+    if (c->precision != SOL_PRECISION_FP64 || c->softening != 0.0) return false;
+    if (c->n < 2 || c->n > kResidentMaxBodies || resident_lds_bytes(c->n) > c->lds_limit) return false;
+    return true;
+}
+
+// Runs accepted steps of integrator ik in the resident kernel: exactly one
+// Driver call when clip is false and max_steps == 1, or the caller loop
+// with DecisionMaking's clipping until the end of the time line (or
+// max_steps steps) when clip is true.  Launches are bounded to about 0.2 s
+// of work each.
+int resident_run(sol_context *c, int ik, double eps, sol_timeline *tl, int64_t max_steps, bool clip, double *body_h,
+                 int64_t *steps_out, int64_t *attempts_out)
+{
+    ResSystem &S = *c->res_h;
+    S = ResSystem{};
+    S.n = c->n;
+    S.nm = c->n_massive;
+    source_ends(c, S.end_a, S.end_b);
+    S.base = 0;
+    S.barycentric = c->barycentric;
+    S.clip = clip ? 1 : 0;
+    S.status = RES_RUNNING;
+    S.millenium = tl->millenium;
+    S.t = tl->time; S.h_next = tl->hNext; S.h_did = tl->hDid; S.eps = eps;
+    S.elapsed = tl->elapsedTime; S.last_save = tl->lastSave; S.last_nsteps = tl->lastNSteps;
+    S.length = tl->length; S.output = tl->output;
+    HIPCHECK(c, hipMemcpyAsync(c->res_d, c->res_h, sizeof(ResSystem), hipMemcpyHostToDevice, c->stream));
+    ResArrays A{};
+    A.tot = c->np;
+    A.y0 = c->y0; A.y = c->y; A.acc0 = c->acc0; A.yscale = c->yscale; A.err = c->error; A.fst = c->fst;
+    A.mass = c->mass; A.rm3 = c->rm3; A.nnd = c->nnd; A.nni = c->nni;
+    const int block = resident_block_size(c->n);
+    const double per_step = 13.0 * (double)c->n * (double)c->n / 4.0e9 + 2.0e-5;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(20000, (int64_t)(0.2 / per_step)));
+    for (;;) {
+        const int64_t k = max_steps > 0 ? std::min(chunk, max_steps - S.steps) : chunk;
+        launch_resident(c->stream, ik, c->res_d, 1, A, (int)k, c->coef, block, c->n);
+        HIPCHECK(c, hipGetLastError());
+        HIPCHECK(c, hipMemcpyAsync(c->res_h, c->res_d, sizeof(ResSystem), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        if (S.status != RES_RUNNING) break;
+        if (max_steps > 0 && S.steps >= max_steps) break;
+    }
+    tl->time = S.t; tl->hNext = S.h_next; tl->hDid = S.h_did;
+    tl->elapsedTime = S.elapsed; tl->lastSave = S.last_save; tl->lastNSteps = S.last_nsteps;
+    if (body_h) *body_h = S.body_h;
+    if (steps_out) *steps_out += S.steps;
+    if (attempts_out) *attempts_out += S.attempts;
+    // the Driver leaves the drag switch on and the migration switches off
+    c->eval_drag = 1;
+    c->eval_mig1 = c->eval_mig2 = 0;
+    if (S.status == RES_UNDERFLOW)
+        return c->fail(SOL_ERR_UNDERFLOW, "Stepsize-underflow occurred during Runge-Kutta-Fehlberg7(8) step!");
+    if (S.status == RES_MAXITER)
+        return c->fail(SOL_ERR_MAXITER,
+                       "An error occurred during Prince-Dormand driver: iteration number exceeded maxIter!");
+    return SOL_OK;
+}
+
 int driver_rkf78(sol_context *c, double eps, sol_timeline *tl, double *body_h, int64_t *attempts)
 {
+    if (use_resident(c)) return resident_run(c, INTEG_RKF78, eps, tl, 1, false, body_h, nullptr, attempts);
     const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = 1.89e-4;
     const double t0 = tl->time;
     double h = tl->hNext;
@@ -505,6 +576,7 @@ int driver_rkf78(sol_context *c, double eps, sol_timeline *tl, double *body_h, i
 
 int driver_rk4(sol_context *c, sol_timeline *tl, double *body_h, int64_t *attempts)
 {
+    if (use_resident(c)) return resident_run(c, INTEG_RK4, 0.0, tl, 1, false, body_h, nullptr, attempts);
     const double t0 = tl->time, h = tl->hNext;
     c->eval_drag = c->eval_mig1 = c->eval_mig2 = 1;
     int r = evaluate_at(c, c->y0, t0, h, c->acc0, EPI_STAGE0, INTEG_RK4, false, false);
@@ -523,6 +595,7 @@ int driver_rk4(sol_context *c, sol_timeline *tl, double *body_h, int64_t *attemp
 
 int driver_dp(sol_context *c, double eps, sol_timeline *tl, double *body_h, int64_t *attempts)
 {
+    if (use_resident(c)) return resident_run(c, INTEG_DP, eps, tl, 1, false, body_h, nullptr, attempts);
     const int maxIter = 10;
     const double t0 = tl->time;
     c->eval_drag = c->eval_mig1 = c->eval_mig2 = 1;
@@ -600,8 +673,11 @@ int sol_create(sol_context **out, const sol_nbodies *nb, int32_t barycentric, in
     };
     if (hipSetDevice(device) != hipSuccess) { c->err = "hipSetDevice failed"; return bail(SOL_ERR_DEVICE); }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) {
         c->cu_count = prop.multiProcessorCount;
+        c->lds_limit = prop.sharedMemPerBlock > 2048 ? prop.sharedMemPerBlock - 2048 : 0;
+    }
+    if (const char *e = std::getenv("SOLARIS_AMD_NO_RESIDENT")) c->resident_disabled = e[0] == '1';
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(SOL_ERR_DEVICE);
@@ -618,9 +694,10 @@ int sol_create(sol_context **out, const sol_nbodies *nb, int32_t barycentric, in
         (r = dalloc(c, &c->rm3, np)) || (r = dalloc(c, &c->nnd, np)) || (r = dalloc(c, &c->nni, np)) ||
         (r = dalloc(c, &c->migc, 3 * np)) || (r = dalloc(c, &c->dragc, 3 * np)) ||
         (r = dalloc(c, &c->errblk, c->nblk)) || (r = dalloc(c, &c->scal, 8)) || (r = dalloc(c, &c->coef, 275)) ||
-        (r = dalloc(c, &c->integ, 11 * np)))
+        (r = dalloc(c, &c->integ, 11 * np)) || (r = dalloc(c, &c->res_d, 1)))
         return bail(r);
-    if (hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->res_h, sizeof(ResSystem), hipHostMallocDefault) != hipSuccess) {
         c->err = "hipHostMalloc failed";
         return bail(SOL_ERR_DEVICE);
     }
@@ -652,6 +729,7 @@ int sol_destroy(sol_context *c)
     if (c->nnr2) (void)hipFree(c->nnr2);
     if (c->nnidx) (void)hipFree(c->nnidx);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
+    if (c->res_h) (void)hipHostFree(c->res_h);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -880,7 +958,9 @@ int sol_integrate(sol_context *c, int32_t integrator, double eps, sol_timeline *
     HIPCHECK(c, hipEventRecord(e0, c->stream));
     int64_t steps = 0, attempts = 0;
     int r = SOL_OK;
-    for (;;) {
+    if (use_resident(c)) {
+        r = resident_run(c, ik, eps, tl, max_steps, true, nullptr, &steps, &attempts);
+    } else for (;;) {
         double bh = 0.0;
         if (ik == INTEG_RKF78) r = driver_rkf78(c, eps, tl, &bh, &attempts);
         else if (ik == INTEG_RK4) r = driver_rk4(c, tl, &bh, &attempts);

@@ -1,0 +1,37 @@
+// resident.h -- per-system descriptors of the workgroup-resident drivers
+// (resident.hip).
+#pragma once
+
+#include "sol_internal.h"
+
+namespace sol {
+
+enum { RES_RUNNING = 0, RES_DONE = 1, RES_UNDERFLOW = 2, RES_MAXITER = 3 };
+constexpr int kResidentMaxBodies = 2000;   // LDS: 32 B per body
+
+// One system: its bodies are [base, base + n) of the concatenated arrays.
+// Time-line fields follow TimeLine (Solaris/TimeLine.h); clip switches on
+// the Simulator's step clipping and small-number flush.
+struct ResSystem {
+    int32_t n, nm, end_a, end_b;
+    int32_t base, barycentric, clip, status;
+    int32_t millenium, pad;
+    double t, h_next, h_did, body_h, eps;
+    double elapsed, last_save, last_nsteps, length, output;
+    int64_t steps, attempts;
+};
+
+// Structure-of-arrays storage over all systems: component c of body b at
+// [c * tot + b]; stage l >= 1 derivatives at fst + (l-1) * 6 * tot.
+struct ResArrays {
+    int64_t tot;
+    double *y0, *y, *acc0, *yscale, *err, *fst, *mass, *rm3, *nnd;
+    int32_t *nni;
+};
+
+int resident_block_size(int n);
+size_t resident_lds_bytes(int nmax);
+void launch_resident(hipStream_t st, int integ, ResSystem *systems, int nsys, const ResArrays &A, int max_steps,
+                     const double *coef, int block, int nmax);
+
+}  // namespace sol

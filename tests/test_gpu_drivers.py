@@ -47,9 +47,20 @@ def replay(scn, integrator, eps, steps, h0):
     return ctx, orc
 
 
-def test_rkf78_c1_replay_100_steps():
+@pytest.fixture(params=["resident", "multikernel"])
+def path(request, monkeypatch):
+    """Small gas-free systems run in the workgroup-resident driver kernel;
+    SOLARIS_AMD_NO_RESIDENT=1 forces the per-stage multi-kernel path."""
+    if request.param == "multikernel":
+        monkeypatch.setenv("SOLARIS_AMD_NO_RESIDENT", "1")
+    else:
+        monkeypatch.delenv("SOLARIS_AMD_NO_RESIDENT", raising=False)
+    return request.param
+
+
+def test_rkf78_c1_replay_100_steps(path):
     scn = S.make_c1()
-    print("[rkf78 c1 replay]", flush=True)
+    print(f"[rkf78 c1 replay, {path}]", flush=True)
     ctx, orc = replay(scn, S.RUNGE_KUTTA_FEHLBERG78, 1e-10, 100, scn.h_next)
     # side outputs of the last evaluation (stage 12 of the accepted attempt)
     a, b = ctx.side_outputs(), orc.side_outputs()
@@ -64,10 +75,10 @@ def test_rkf78_c2_replay_100_steps():
     assert np.array_equal(ctx.side_outputs()[3], orc.side_outputs()[3])
 
 
-def test_rk4_c5_100_steps():
+def test_rk4_c5_100_steps(path):
     # fixed step: no step-size control, the two runs take identical steps
     scn = S.make_c5(n_tp=256)
-    print("[rk4 c5]", flush=True)
+    print(f"[rk4 c5, {path}]", flush=True)
     replay(scn, S.RUNGE_KUTTA4, 1e-10, 100, 1.0)
 
 
@@ -95,6 +106,12 @@ def barycentric_invariants(y, m):
         pe -= S.GAUSS2 * (mm[i] * mm[i + 1:] / d).sum()
     L = (mm[:, None] * np.cross(R, Vv)).sum(0)
     return ke + pe, L
+
+
+def test_dormand_prince_c5_replay_resident(path):
+    scn = S.make_c5(n_tp=200)
+    print(f"[dp c5 replay, {path}]", flush=True)
+    replay(scn, S.DORMAND_PRINCE, 1e-10, 40, 5.0)
 
 
 def test_rkf78_c1_free_running():
@@ -153,7 +170,7 @@ def test_step_and_error_max_match():
         assert np.abs(eg - eo).max() <= 1e-4 * np.abs(eo).max(), (integ, np.abs(eg - eo).max(), np.abs(eo).max())
 
 
-def test_integrate_clips_to_length_and_output():
+def test_integrate_clips_to_length_and_output(path):
     scn = S.make_c1(n_tp=32)
     ctx, orc = runtime.Context(scn), Oracle(scn)
     ctx.set_state(scn.y0); orc.set_state(scn.y0)
