@@ -260,6 +260,35 @@ int sol_profile(sol_context *ctx, int32_t enable, sol_kernel_profile *out);
  * sum over sinks i >= 1 of (1 + |S(i) \ {i}|). */
 int sol_interactions_per_evaluation(const sol_context *ctx, double *count);
 
+/* ---- batched ensemble ------------------------------------------------- */
+/* Many independent systems (each a star plus planets, no gas) integrated
+ * together: one workgroup per system runs whole Driver loops with the
+ * caller loop's clipping, systems of similar size share a launch whose block
+ * shape is picked by timing, and the launches of one round are replayed
+ * from a captured HIP graph.  No counterpart in the reference, where each
+ * system is a separate simulation (Simulator::Integrate,
+ * Solaris/Simulator.cpp:121-178); per system the semantics are those of
+ * sol_integrate.  A failing system stops with its own status. */
+typedef struct sol_ensemble sol_ensemble;
+enum { SOL_SYSTEM_RUNNING = 0, SOL_SYSTEM_DONE = 1, SOL_SYSTEM_UNDERFLOW = 2, SOL_SYSTEM_MAXITER = 3 };
+
+int sol_ensemble_create(sol_ensemble **ens, int32_t nsys, const sol_nbodies *nbodies,
+                        int32_t device);
+int sol_ensemble_destroy(sol_ensemble *ens);
+const char *sol_ensemble_last_error(const sol_ensemble *ens);
+/* System k: per-body parameters, initial state (6*total doubles, AoS) and
+ * its time line (time, hNext, length, output). */
+int sol_ensemble_set_system(sol_ensemble *ens, int32_t k, const sol_body_params *params,
+                            const double *y0, const sol_timeline *tl);
+/* Integrates every running system until it reaches the end of its time
+ * line or max_steps accepted steps in this call (max_steps <= 0: no
+ * limit).  stats (may be NULL) sums over systems. */
+int sol_ensemble_run(sol_ensemble *ens, int32_t integrator, double epsilon, int64_t max_steps,
+                     sol_run_stats *stats);
+/* State (6*total doubles), time line and status (SOL_SYSTEM_*) of system k. */
+int sol_ensemble_get_system(sol_ensemble *ens, int32_t k, double *y0, sol_timeline *tl,
+                            int32_t *status, int64_t *steps);
+
 /* Library version string. */
 const char *sol_version(void);
 

@@ -20,13 +20,11 @@
 
 using namespace sol;
 
-namespace {
-
 // Fehlberg 7(8) coefficients D_{s,l} (NASA TR R-381; as set in
 // RungeKuttaFehlberg78.cpp:41-56), RK4 tableau (RungeKutta4.cpp:60-70) and
 // Dormand-Prince RKN 7(6) (DormandPrince.cpp:37-123), packed at the offsets
-// stages.hip reads.
-void fill_coefficients(std::vector<double> &c)
+// stages.hip and resident.hip read.
+void sol::fill_coefficients(std::vector<double> &c)
 {
     c.assign(275, 0.0);
     const double D[13][12] = {
@@ -81,6 +79,8 @@ void fill_coefficients(std::vector<double> &c)
                           1.0, 1.0};
     for (int k = 0; k < 9; ++k) { c[248 + k] = b[k]; c[257 + k] = bdh[k]; c[266 + k] = cc[k]; }
 }
+
+namespace {
 
 double host_reduction_factor(const sol_gas_component &g, double t)
 {   // GasComponent::ReductionFactor, Solaris/GasComponent.cpp:36-61

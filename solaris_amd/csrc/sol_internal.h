@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/solaris_amd.h"
 
 namespace sol {
@@ -93,6 +95,10 @@ struct EpiArgs {
     const double *scal;       // device scalars: [0] = h, [1] = t
     const double *coef;       // integrator coefficient tables (context.cpp)
 };
+
+// Integrator coefficient tables packed at the offsets the kernels read
+// (275 doubles).
+void fill_coefficients(std::vector<double> &c);
 
 // Host-side launchers (pairs.hip / stages.hip).
 void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,

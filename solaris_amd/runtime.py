@@ -59,6 +59,14 @@ _SIGS = {
     "sol_interactions_per_evaluation": (C.c_int, [V, D]),
     "sol_version": (C.c_char_p, []),
     "sol_profile": (C.c_int, [V, C.c_int32, C.POINTER(_abi.KernelProfile)]),
+    "sol_ensemble_create": (C.c_int, [C.POINTER(V), C.c_int32, C.POINTER(_abi.NBodies), C.c_int32]),
+    "sol_ensemble_destroy": (C.c_int, [V]),
+    "sol_ensemble_last_error": (C.c_char_p, [V]),
+    "sol_ensemble_set_system": (C.c_int, [V, C.c_int32, C.POINTER(_abi.BodyParams), D,
+                                          C.POINTER(_abi.TimeLine)]),
+    "sol_ensemble_run": (C.c_int, [V, C.c_int32, C.c_double, C.c_int64, C.POINTER(_abi.RunStats)]),
+    "sol_ensemble_get_system": (C.c_int, [V, C.c_int32, D, C.POINTER(_abi.TimeLine), I,
+                                          C.POINTER(C.c_int64)]),
 }
 
 
@@ -224,3 +232,53 @@ class Context:
 
     def last_error(self):
         return self.lib.sol_last_error(self.h).decode()
+
+
+class Ensemble:
+    """Independent systems integrated together (sol_ensemble_*)."""
+
+    def __init__(self, scenarios, device: int = 0):
+        self.lib = library()
+        self.scns = list(scenarios)
+        arr = (_abi.NBodies * len(self.scns))(*[_abi.nbodies(s.counts) for s in self.scns])
+        h = V()
+        r = self.lib.sol_ensemble_create(C.byref(h), len(self.scns), arr, int(device))
+        if r:
+            raise SolarisError(r, self.lib.sol_ensemble_last_error(None).decode())
+        self.h = h
+
+    def _check(self, r):
+        if r:
+            raise SolarisError(r, self.lib.sol_ensemble_last_error(self.h).decode())
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sol_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_system(self, k, scn, tl):
+        keep, bp = _abi.body_params(scn)
+        y = np.ascontiguousarray(scn.y0, dtype=np.float64).reshape(-1)
+        self._check(self.lib.sol_ensemble_set_system(self.h, int(k), C.byref(bp), _dp(y), C.byref(tl)))
+
+    def run(self, integrator, eps, max_steps=0):
+        st = _abi.RunStats()
+        self._check(self.lib.sol_ensemble_run(self.h, int(integrator), float(eps), int(max_steps), C.byref(st)))
+        return st
+
+    def get_system(self, k):
+        n = self.scns[k].n
+        y = np.zeros(6 * n)
+        tl = _abi.TimeLine()
+        status = C.c_int32(0)
+        steps = C.c_int64(0)
+        self._check(self.lib.sol_ensemble_get_system(self.h, int(k), _dp(y), C.byref(tl), C.byref(status),
+                                                     C.byref(steps)))
+        return y.reshape(n, 6), tl, status.value, steps.value
