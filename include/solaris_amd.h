@@ -289,6 +289,30 @@ int sol_ensemble_run(sol_ensemble *ens, int32_t integrator, double epsilon, int6
 int sol_ensemble_get_system(sol_ensemble *ens, int32_t k, double *y0, sol_timeline *tl,
                             int32_t *status, int64_t *steps);
 
+/* ---- adjoint of fixed-step RK4 ---------------------------------------- */
+/* Forward pass to record: nsteps RungeKutta4::Step calls
+ * (Solaris/RungeKutta4.cpp:58-130) chained from the device Y0 at time t0,
+ * each step starting with the force evaluation at its own state (as
+ * RungeKutta4::Driver does, Solaris/RungeKutta4.cpp:20-56), step i of size
+ * h_seq[i] (or h when h_seq is NULL).  The step sequence is frozen and the
+ * state is kept on the device every checkpoint_every steps.  Gravity only,
+ * astrocentric, FP64, no softening: a context with a nebula, barycentric
+ * frame, mixed precision or softening is rejected with
+ * SOL_ERR_UNSUPPORTED.  Y0 is left unchanged and Y receives the final
+ * state; y_final (may be NULL) gets it too (6*total doubles).  No
+ * counterpart in the reference, which has no gradients. */
+int sol_adjoint_forward(sol_context *ctx, double t0, double h, const double *h_seq,
+                        int64_t nsteps, int32_t checkpoint_every, double *y_final);
+/* Reverse pass over the recorded steps: given dL/dy_final (6*total
+ * doubles, AoS) returns dL/dy0 (6*total) and dL/dm (total doubles, per
+ * body mass in solar masses; the central body's entry included).  Each
+ * segment between checkpoints is recomputed forward, then the discrete
+ * adjoint of the RK4 stages runs backwards through it with the force
+ * vector-Jacobian product evaluated on the device.  Either output may be
+ * NULL.  No counterpart in the reference. */
+int sol_adjoint_backward(sol_context *ctx, const double *grad_y_final, double *grad_y0,
+                         double *grad_mass);
+
 /* Library version string. */
 const char *sol_version(void);
 

@@ -1003,3 +1003,83 @@ void cr_integrals(const cr_system *s, double *out)
     pe *= 0.5*C_GAUSS2;
     out[13] = ke; out[14] = pe; out[15] = ke - pe;
 }
+
+/* ---- forward-mode derivative of fixed-step RK4 (test reference for the
+ * adjoint) ------------------------------------------------------------------
+ * Dual numbers (value, derivative) carried through GravityAC
+ * (Solaris/Acceleration.cpp:248-329, gravity only) and RungeKutta4::Step
+ * (Solaris/RungeKutta4.cpp:58-130): the derivative of the final state along
+ * one direction (dy0 on the initial state, dm on the masses). */
+typedef struct { double v, d; } dual;
+static dual dl(double v, double d) { dual r = {v, d}; return r; }
+static dual dadd(dual a, dual b) { return dl(a.v + b.v, a.d + b.d); }
+static dual dsub(dual a, dual b) { return dl(a.v - b.v, a.d - b.d); }
+static dual dmul(dual a, dual b) { return dl(a.v * b.v, a.d * b.v + a.v * b.d); }
+static dual ddiv(dual a, dual b) { return dl(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+static dual dsqrt(dual a) { double s = sqrt(a.v); return dl(s, a.d / (2.0 * s)); }
+static dual dscale(double c, dual a) { return dl(c * a.v, c * a.d); }
+
+static void gravity_ac_dual(const cr_system *s, const dual *y, const dual *m, dual *a)
+{
+    const int n = s->n;
+    dual *rm3 = malloc(sizeof(dual) * n);
+    for (int i = 1; i < n; i++) {
+        const dual *p = y + 6 * i;
+        dual r2 = dadd(dadd(dmul(p[0], p[0]), dmul(p[1], p[1])), dmul(p[2], p[2]));
+        rm3[i] = ddiv(dl(1.0, 0.0), dmul(r2, dsqrt(r2)));
+    }
+    for (int k = 0; k < 6; k++) a[k] = dl(0.0, 0.0);
+    for (int i = 1; i < n; i++) {
+        const int i0 = 6 * i;
+        dual mu = dscale(C_GAUSS2, dadd(m[0], m[i]));
+        a[i0 + 0] = y[i0 + 3]; a[i0 + 1] = y[i0 + 4]; a[i0 + 2] = y[i0 + 5];
+        dual acc[3];
+        for (int c = 0; c < 3; c++) acc[c] = dmul(dmul(dscale(-1.0, mu), rm3[i]), y[i0 + c]);
+        const int end = src_end_ac(s, i);
+        dual px = dl(0, 0), py = dl(0, 0), pz = dl(0, 0);
+        for (int j = 1; j < end; j++) {
+            if (j == i) continue;
+            const int j0 = 6 * j;
+            dual dx = dsub(y[j0], y[i0]), dy = dsub(y[j0 + 1], y[i0 + 1]), dz = dsub(y[j0 + 2], y[i0 + 2]);
+            dual r2 = dadd(dadd(dmul(dx, dx), dmul(dy, dy)), dmul(dz, dz));
+            dual rijm3 = ddiv(dl(1.0, 0.0), dmul(r2, dsqrt(r2)));
+            dual gm = dscale(C_GAUSS2, m[j]);
+            px = dadd(px, dmul(gm, dsub(dmul(dx, rijm3), dmul(y[j0], rm3[j]))));
+            py = dadd(py, dmul(gm, dsub(dmul(dy, rijm3), dmul(y[j0 + 1], rm3[j]))));
+            pz = dadd(pz, dmul(gm, dsub(dmul(dz, rijm3), dmul(y[j0 + 2], rm3[j]))));
+        }
+        a[i0 + 3] = dadd(acc[0], px); a[i0 + 4] = dadd(acc[1], py); a[i0 + 5] = dadd(acc[2], pz);
+    }
+    free(rm3);
+}
+
+int cr_rk4_tangent(const cr_system *s, const double *y0, const double *dy0, const double *dm,
+                   double h_const, const double *h_seq, int64_t nsteps, double *yT, double *dyT)
+{
+    const int n = s->n, nv = 6 * n;
+    dual *y = malloc(sizeof(dual) * nv), *yt = malloc(sizeof(dual) * nv), *m = malloc(sizeof(dual) * n);
+    dual *k[4];
+    for (int q = 0; q < 4; q++) k[q] = malloc(sizeof(dual) * nv);
+    for (int i = 0; i < nv; i++) y[i] = dl(y0[i], dy0 ? dy0[i] : 0.0);
+    for (int i = 0; i < n; i++) m[i] = dl(s->mass[i], dm ? dm[i] : 0.0);
+    const double b1 = 1.0 / 6.0, b2 = 1.0 / 3.0, b3 = 1.0 / 3.0, b4 = 1.0 / 6.0;
+    for (int64_t st = 0; st < nsteps; st++) {
+        const double h = h_seq ? h_seq[st] : h_const;
+        gravity_ac_dual(s, y, m, k[0]);
+        for (int i = 0; i < nv; i++) yt[i] = dadd(y[i], dscale(h, dscale(0.5, k[0][i])));
+        gravity_ac_dual(s, yt, m, k[1]);
+        for (int i = 0; i < nv; i++) yt[i] = dadd(y[i], dscale(h, dscale(0.5, k[1][i])));
+        gravity_ac_dual(s, yt, m, k[2]);
+        for (int i = 0; i < nv; i++) yt[i] = dadd(y[i], dscale(h, dscale(1.0, k[2][i])));
+        gravity_ac_dual(s, yt, m, k[3]);
+        for (int i = 0; i < nv; i++) {
+            dual acc = dadd(dadd(dadd(dscale(b1, k[0][i]), dscale(b2, k[1][i])), dscale(b3, k[2][i])),
+                            dscale(b4, k[3][i]));
+            y[i] = dadd(y[i], dscale(h, acc));
+        }
+    }
+    for (int i = 0; i < nv; i++) { yT[i] = y[i].v; dyT[i] = y[i].d; }
+    for (int q = 0; q < 4; q++) free(k[q]);
+    free(y); free(yt); free(m);
+    return 0;
+}

@@ -49,6 +49,12 @@ int        cr_integrate(cr_system *s, int32_t integrator, double epsilon,
                         sol_timeline *tl, int64_t max_steps, sol_run_stats *stats);
 void       cr_integrals(const cr_system *s, double *out16);
 double     cr_interactions_per_evaluation(const cr_system *s);
+/* Forward-mode (dual-number) derivative of nsteps RK4 steps from y0, step
+ * i of size h_seq[i] (h_const when h_seq is NULL), astrocentric gravity
+ * only, no softening: yT and its derivative dyT along the direction
+ * (dy0, dm) (either may be NULL; dm is per body mass in solar masses). */
+int        cr_rk4_tangent(const cr_system *s, const double *y0, const double *dy0, const double *dm,
+                          double h_const, const double *h_seq, int64_t nsteps, double *yT, double *dyT);
 
 /* Single gas/orbit helpers, exposed for the reference's own unit-test
  * vectors (src/Solaris.NBody.Cuda.Test/unit_test.cpp:558-699,

@@ -71,6 +71,7 @@ def _load(kind: str):
             "orbital_a_e": (C.c_int, [C.c_double, D, D, D]),
             "default_gas": (None, [C.POINTER(_abi.GasComponent)]),
             "constant": (C.c_double, [C.c_char_p]),
+            "rk4_tangent": (C.c_int, [V, D, D, D, C.c_double, D, C.c_int64, D, D]),
         })
     else:
         sig.update({
@@ -210,6 +211,20 @@ class Oracle(_Base):
 
     def interactions_per_evaluation(self):
         return self.f["interactions_per_evaluation"](self.h)
+
+    def rk4_tangent(self, y0, h, nsteps, dy0=None, dm=None, h_seq=None):
+        """Forward-mode derivative of nsteps RK4 steps (gravity only): the final
+        state and its derivative along (dy0, dm)."""
+        y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1)
+        h_seq = None if h_seq is None else np.ascontiguousarray(h_seq, dtype=np.float64)
+        yT = np.zeros_like(y0)
+        dyT = np.zeros_like(y0)
+        dy0 = None if dy0 is None else np.ascontiguousarray(dy0, dtype=np.float64).reshape(-1)
+        dm = None if dm is None else np.ascontiguousarray(dm, dtype=np.float64)
+        self.f["rk4_tangent"](self.h, _dp(y0), None if dy0 is None else _dp(dy0),
+                              None if dm is None else _dp(dm), float(h),
+                              None if h_seq is None else _dp(h_seq), int(nsteps), _dp(yT), _dp(dyT))
+        return yT.reshape(-1, 6), dyT.reshape(-1, 6)
 
 
 class Reference(_Base):

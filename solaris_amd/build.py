@@ -29,7 +29,8 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
-HIP_SOURCES = ["pairs.hip", "stages.hip", "resident.hip", "context.cpp", "ensemble.cpp"]
+HIP_SOURCES = ["pairs.hip", "stages.hip", "resident.hip", "adjoint.hip", "context.cpp", "ensemble.cpp",
+               "adjoint.cpp"]
 
 
 def _run(cmd, quiet=True):
@@ -48,7 +49,7 @@ def build_hip(verbose: bool = False) -> str:
     objs = []
     cmds = []
     for src in HIP_SOURCES:
-        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(BUILD, src.replace(".", "_") + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

@@ -1,0 +1,311 @@
+// adjoint.cpp -- reverse-mode gradient of fixed-step RK4 (gravity only).
+//
+// Forward (sol_adjoint_forward): nsteps RK4 steps (RungeKutta4::Step,
+// Solaris/RungeKutta4.cpp:58-130, each with its own f0 = Compute(y0) as
+// RungeKutta4::Driver does, Solaris/RungeKutta4.cpp:20-56) from the device
+// Y0; the step sizes are frozen and the state is stored every K steps.
+//
+// Backward (sol_adjoint_backward): for each segment of K steps, last first,
+// the states of the segment and their stage states Y1..Y4 are recomputed
+// from its checkpoint; then, step by step backwards, the discrete adjoint
+// of the step y' = y + h (k1/6 + k2/3 + k3/3 + k4/6),
+//   Y1 = y, Y2 = y + h/2 k1, Y3 = y + h/2 k2, Y4 = y + h k3, k_s = f(Y_s):
+//     kb4 = h/6 lam;              G4 = J(Y4)^T kb4
+//     kb3 = h/3 lam + h G4;       G3 = J(Y3)^T kb3
+//     kb2 = h/3 lam + h/2 G3;     G2 = J(Y2)^T kb2
+//     kb1 = h/6 lam + h/2 G2;     G1 = J(Y1)^T kb1
+//     lam <- lam + G1 + G2 + G3 + G4,  dL/dm += sum_s (df/dm)(Y_s)^T kb_s,
+// with the force vector-Jacobian products of adjoint.hip.  Memory: the
+// checkpoints (ceil(nsteps/K) x 6 NP doubles) and one segment of stage
+// states (4 K x 6 NP doubles).
+#include <cmath>
+#include <cstring>
+
+#include "context.h"
+
+namespace sol {
+
+struct AdjointState {
+    int64_t nsteps = 0;
+    int K = 0;
+    bool recorded = false;
+    std::vector<double> h, t;          // step sizes and start times
+    size_t ncp = 0, ckpt_cap = 0;      // checkpoints held / allocated
+    double *ckpt = nullptr;            // [ncp][6 NP]
+    int seg_cap = 0;
+    double *seg = nullptr;             // [K][4][6 NP] stage states of one segment
+    double *kb = nullptr;              // [4][6 NP] stage derivatives
+    double *ya = nullptr, *yb = nullptr, *ytmp = nullptr;
+    double *lam = nullptr, *ybar = nullptr, *w0 = nullptr, *w1 = nullptr;
+    double *mbar = nullptr, *m0blk = nullptr, *mout = nullptr;
+    double *psink = nullptr, *psrc = nullptr;
+    int psink_cap = 0, psrc_cap = 0;   // chunks allocated
+    VjpArgs shape{};                   // launch shape of the products
+    std::vector<void *> allocs;
+};
+
+void adjoint_release(sol_context *c)
+{
+    if (!c || !c->adj) return;
+    for (void *p : c->adj->allocs) (void)hipFree(p);
+    if (c->adj->ckpt) (void)hipFree(c->adj->ckpt);
+    if (c->adj->seg) (void)hipFree(c->adj->seg);
+    if (c->adj->psink) (void)hipFree(c->adj->psink);
+    if (c->adj->psrc) (void)hipFree(c->adj->psrc);
+    delete c->adj;
+    c->adj = nullptr;
+}
+
+void adjoint_invalidate(sol_context *c)
+{
+    if (c && c->adj) c->adj->recorded = false;
+}
+
+}  // namespace sol
+
+using namespace sol;
+
+namespace {
+
+int amalloc(sol_context *c, AdjointState *A, double **p, size_t count)
+{
+    void *q = nullptr;
+    HIPCHECK(c, hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(double)));
+    HIPCHECK(c, hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(double)));
+    A->allocs.push_back(q);
+    *p = static_cast<double *>(q);
+    return SOL_OK;
+}
+
+// Launch shape of the vector-Jacobian products: sinks [1, n) in tiles of 64,
+// sources [1, jmax); the other dimension is cut into chunks so that a launch
+// has about 8 waves per CU.
+void vjp_shape(const sol_context *c, VjpArgs &v)
+{
+    v = VjpArgs{};
+    v.n = c->n;
+    v.np = c->np;
+    v.n_massive = c->n_massive;
+    source_ends(c, v.ea, v.eb);
+    v.jmax = std::min(c->n, std::max(std::max(v.ea, v.eb), 1));
+    const int waves = std::max(c->cu_count, 1) * 8;
+    const int nsink = c->n - 1, nsrc = v.jmax - 1;
+    if (nsink > 0 && nsrc > 0) {
+        v.tiles_s = (nsink + kWave - 1) / kWave;
+        v.nch_s = std::max(1, std::min((waves + v.tiles_s - 1) / v.tiles_s, (nsrc + kWave - 1) / kWave));
+        v.ch_s = (nsrc + v.nch_s - 1) / v.nch_s;
+        v.nch_s = (nsrc + v.ch_s - 1) / v.ch_s;
+        v.tiles_b = (nsrc + kWave - 1) / kWave;
+        v.nch_b = std::max(1, std::min((waves + v.tiles_b - 1) / v.tiles_b, (nsink + kWave - 1) / kWave));
+        v.ch_b = (nsink + v.nch_b - 1) / v.nch_b;
+        v.nch_b = (nsink + v.ch_b - 1) / v.ch_b;
+    }
+}
+
+int ensure_state(sol_context *c, int K, size_t ncp)
+{
+    if (!c->adj) {
+        AdjointState *A = new AdjointState();
+        c->adj = A;
+        const size_t v = 6 * (size_t)c->np;
+        int r;
+        if ((r = amalloc(c, A, &A->kb, 4 * v)) || (r = amalloc(c, A, &A->ya, v)) || (r = amalloc(c, A, &A->yb, v)) ||
+            (r = amalloc(c, A, &A->ytmp, v)) || (r = amalloc(c, A, &A->lam, v)) || (r = amalloc(c, A, &A->ybar, v)) ||
+            (r = amalloc(c, A, &A->w0, v)) || (r = amalloc(c, A, &A->w1, v)) || (r = amalloc(c, A, &A->mbar, c->np)) ||
+            (r = amalloc(c, A, &A->m0blk, c->nblk)) || (r = amalloc(c, A, &A->mout, c->np))) {
+            adjoint_release(c);
+            return r;
+        }
+    }
+    AdjointState *A = c->adj;
+    const size_t v = 6 * (size_t)c->np;
+    // the source ranges may have changed since the last call
+    vjp_shape(c, A->shape);
+    if (A->shape.nch_s > A->psink_cap) {
+        if (A->psink) (void)hipFree(A->psink);
+        A->psink = nullptr;
+        A->psink_cap = 0;
+        HIPCHECK(c, hipMalloc(&A->psink, (size_t)A->shape.nch_s * 3 * c->np * sizeof(double)));
+        A->psink_cap = A->shape.nch_s;
+    }
+    if (A->shape.nch_b > A->psrc_cap) {
+        if (A->psrc) (void)hipFree(A->psrc);
+        A->psrc = nullptr;
+        A->psrc_cap = 0;
+        HIPCHECK(c, hipMalloc(&A->psrc, (size_t)A->shape.nch_b * 7 * c->np * sizeof(double)));
+        A->psrc_cap = A->shape.nch_b;
+    }
+    if (ncp > A->ckpt_cap) {
+        if (A->ckpt) (void)hipFree(A->ckpt);
+        A->ckpt = nullptr;
+        A->ckpt_cap = 0;
+        HIPCHECK(c, hipMalloc(&A->ckpt, ncp * v * sizeof(double)));
+        A->ckpt_cap = ncp;
+    }
+    if (K > A->seg_cap) {
+        if (A->seg) (void)hipFree(A->seg);
+        A->seg = nullptr;
+        A->seg_cap = 0;
+        HIPCHECK(c, hipMalloc(&A->seg, (size_t)K * 4 * v * sizeof(double)));
+        A->seg_cap = K;
+    }
+    return SOL_OK;
+}
+
+// One RK4 step from y at (t, h) into ynext.  With stages set, the stage
+// states Y1..Y4 are kept there ([4][6 NP]).
+int rk4_step(sol_context *c, AdjointState *A, const double *y, double t, double h, double *stages, double *ynext)
+{
+    const size_t v = 6 * (size_t)c->np;
+    double *Y2 = stages ? stages + v : A->ytmp;
+    double *Y3 = stages ? stages + 2 * v : A->ytmp;
+    double *Y4 = stages ? stages + 3 * v : A->ytmp;
+    if (stages) HIPCHECK(c, hipMemcpyAsync(stages, y, v * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    int r;
+    if ((r = eval_force(c, y, t, A->kb))) return r;
+    launch_adj_stage(c->stream, v, y, A->kb, 0.5, h, Y2);
+    if ((r = eval_force(c, Y2, t + 0.5 * h, A->kb + v))) return r;
+    launch_adj_stage(c->stream, v, y, A->kb + v, 0.5, h, Y3);
+    if ((r = eval_force(c, Y3, t + 0.5 * h, A->kb + 2 * v))) return r;
+    launch_adj_stage(c->stream, v, y, A->kb + 2 * v, 1.0, h, Y4);
+    if ((r = eval_force(c, Y4, t + h, A->kb + 3 * v))) return r;
+    if (ynext) launch_adj_final(c->stream, v, y, A->kb, v, h, ynext);
+    HIPCHECK(c, hipGetLastError());
+    return SOL_OK;
+}
+
+int check_supported(sol_context *c)
+{
+    if (c->barycentric) return c->fail(SOL_ERR_UNSUPPORTED, "adjoint: barycentric frame not supported");
+    if (c->has_gas) return c->fail(SOL_ERR_UNSUPPORTED, "adjoint: gas terms not supported (remove the nebula)");
+    if (c->precision != SOL_PRECISION_FP64 || c->softening != 0.0)
+        return c->fail(SOL_ERR_UNSUPPORTED, "adjoint: FP64 without softening only");
+    return SOL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sol_adjoint_forward(sol_context *c, double t0, double h, const double *h_seq, int64_t nsteps,
+                        int32_t checkpoint_every, double *y_final)
+{
+    if (!c) return SOL_ERR_ARGUMENT;
+    if (nsteps <= 0 || checkpoint_every <= 0)
+        return c->fail(SOL_ERR_ARGUMENT, "adjoint: nsteps and checkpoint_every must be positive");
+    int r = check_supported(c);
+    if (r) return r;
+    HIPCHECK(c, hipSetDevice(c->device));
+    std::vector<double> hs(nsteps), ts(nsteps);
+    double t = t0;
+    for (int64_t i = 0; i < nsteps; ++i) {
+        hs[i] = h_seq ? h_seq[i] : h;
+        if (!std::isfinite(hs[i])) return c->fail(SOL_ERR_ARGUMENT, "adjoint: step %lld is not finite", (long long)i);
+        ts[i] = t;
+        t += hs[i];
+    }
+    const int K = (int)std::min<int64_t>(checkpoint_every, nsteps);
+    const size_t ncp = (size_t)((nsteps + K - 1) / K);
+    if ((r = ensure_state(c, K, ncp))) return r;
+    AdjointState *A = c->adj;
+    A->recorded = false;
+    const size_t v = 6 * (size_t)c->np;
+    HIPCHECK(c, hipMemcpyAsync(A->ya, c->y0, v * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    double *cur = A->ya, *nxt = A->yb;
+    for (int64_t i = 0; i < nsteps; ++i) {
+        if (i % K == 0)
+            HIPCHECK(c, hipMemcpyAsync(A->ckpt + (size_t)(i / K) * v, cur, v * sizeof(double),
+                                       hipMemcpyDeviceToDevice, c->stream));
+        if ((r = rk4_step(c, A, cur, ts[i], hs[i], nullptr, nxt))) return r;
+        std::swap(cur, nxt);
+    }
+    HIPCHECK(c, hipMemcpyAsync(c->y, cur, v * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (y_final) {
+        launch_soa_to_aos(c->stream, c->n, c->np, cur, c->aos);
+        HIPCHECK(c, hipMemcpyAsync(y_final, c->aos, 6 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    A->nsteps = nsteps;
+    A->K = K;
+    A->ncp = ncp;
+    A->h.swap(hs);
+    A->t.swap(ts);
+    A->recorded = true;
+    return SOL_OK;
+}
+
+int sol_adjoint_backward(sol_context *c, const double *grad_y_final, double *grad_y0, double *grad_mass)
+{
+    if (!c) return SOL_ERR_ARGUMENT;
+    if (!grad_y_final) return c->fail(SOL_ERR_ARGUMENT, "adjoint: null gradient");
+    if (!c->adj || !c->adj->recorded) return c->fail(SOL_ERR_ARGUMENT, "adjoint: no recorded forward pass");
+    int r = check_supported(c);
+    if (r) return r;
+    HIPCHECK(c, hipSetDevice(c->device));
+    AdjointState *A = c->adj;
+    const size_t v = 6 * (size_t)c->np;
+    // lam = dL/dy_final in SoA
+    HIPCHECK(c, hipMemcpyAsync(c->aos, grad_y_final, 6 * (size_t)c->n * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipMemsetAsync(A->lam, 0, v * sizeof(double), c->stream));
+    launch_aos_to_soa(c->stream, c->n, c->np, c->aos, A->lam);
+    HIPCHECK(c, hipMemsetAsync(A->mbar, 0, (size_t)c->np * sizeof(double), c->stream));
+    HIPCHECK(c, hipMemsetAsync(A->m0blk, 0, (size_t)c->nblk * sizeof(double), c->stream));
+    VjpArgs a = A->shape;
+    a.gm = c->gm;
+    a.psink = A->psink;
+    a.psrc = A->psrc;
+    a.mbar = A->mbar;
+    a.m0blk = A->m0blk;
+    double *lam = A->lam, *ybar = A->ybar;
+    for (int64_t sg = (int64_t)A->ncp - 1; sg >= 0; --sg) {
+        const int64_t n0 = sg * A->K, n1 = std::min<int64_t>(A->nsteps, n0 + A->K);
+        // recompute the segment's stage states from its checkpoint
+        const double *cur = A->ckpt + (size_t)sg * v;
+        double *bufs[2] = {A->ya, A->yb};
+        for (int64_t i = n0; i < n1; ++i) {
+            double *stages = A->seg + (size_t)(i - n0) * 4 * v;
+            double *nxt = (i + 1 < n1) ? bufs[(i - n0) & 1] : nullptr;
+            if ((r = rk4_step(c, A, cur, A->t[i], A->h[i], stages, nxt))) return r;
+            cur = nxt;
+        }
+        for (int64_t i = n1 - 1; i >= n0; --i) {
+            const double h = A->h[i];
+            const double *stages = A->seg + (size_t)(i - n0) * 4 * v;
+            double *w = A->w0, *wn = A->w1;
+            launch_adj_init(c->stream, v, lam, ybar, w, h / 6.0);
+            a.lam = lam;
+            a.ybar = ybar;
+            // stage s = 4, 3, 2, 1: weight of k_s, and the next stage's weight
+            const double alpha[4] = {h / 3.0, h / 3.0, h / 6.0, 0.0};
+            const double beta[4] = {h, 0.5 * h, 0.5 * h, 0.0};
+            for (int q = 0; q < 4; ++q) {
+                a.Y = stages + (size_t)(3 - q) * v;
+                a.W = w;
+                a.wnext = q < 3 ? wn : nullptr;
+                a.alpha = alpha[q];
+                a.beta = beta[q];
+                launch_vjp(c->stream, a);
+                std::swap(w, wn);
+            }
+            HIPCHECK(c, hipGetLastError());
+            std::swap(lam, ybar);
+        }
+    }
+    if (grad_y0) {
+        launch_soa_to_aos(c->stream, c->n, c->np, lam, c->aos);
+        HIPCHECK(c, hipMemcpyAsync(grad_y0, c->aos, 6 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    if (grad_mass) {
+        launch_adj_mass(c->stream, c->n, A->mbar, A->m0blk, c->nblk, A->mout);
+        HIPCHECK(c, hipMemcpyAsync(grad_mass, A->mout, (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return SOL_OK;
+}
+
+}  // extern "C"

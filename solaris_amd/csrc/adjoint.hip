@@ -1,0 +1,274 @@
+// adjoint.hip -- device kernels of the RK4 adjoint (adjoint.cpp).
+//
+// Force model: astrocentric gravity (Solaris/Acceleration.cpp:248-329),
+//   a_i = -mu_i r_i / |r_i|^3 + sum_{j in S(i), j != i} gm_j (d_ij / |d_ij|^3 - r_j / |r_j|^3),
+//   d_ij = r_j - r_i, mu_i = gm_0 + gm_i, S(i) = [1, end(i)), end(i) = ea for
+//   sinks i < n_massive and eb for the others; the central body's
+//   derivative is zero.
+// With T(x) = I/|x|^3 - 3 x x^T/|x|^5 (symmetric, even in x):
+//   d a_i / d r_i = -mu_i T(r_i) - sum_j gm_j T(d_ij)
+//   d a_i / d r_j = gm_j (T(d_ij) - T(r_j))
+//   d a_i / d gm_0 = d a_i / d gm_i = -r_i / |r_i|^3
+//   d a_i / d gm_j = d_ij / |d_ij|^3 - r_j / |r_j|^3
+// The vector-Jacobian product of f = (v, a) with a weight w = (w_r, u)
+// (u the weight on a) is then, for body k >= 1,
+//   G_v,k  = w_r,k
+//   G_r,k  = -mu_k T(r_k) u_k - sum_{j in S(k)} gm_j T(d_kj) u_k          (sink part)
+//            + gm_k (A_k - T(r_k) U_k)                                   (source part)
+//   dL/dgm_k += -u_k.r_k/|r_k|^3 + B_k - U_k.r_k/|r_k|^3
+//   dL/dgm_0 += sum_k -u_k.r_k/|r_k|^3
+// with, over the sinks i that see k (k in S(i), i != k):
+//   A_k = sum_i T(r_k - r_i) u_i,  B_k = sum_i u_i.(r_k - r_i)/|r_k - r_i|^3,
+//   U_k = sum_i u_i.
+// k_vjp_sink forms the sink part, k_vjp_src the sums A, B, U (both as
+// per-chunk partials, so that a launch has enough waves whatever the
+// shape), and k_vjp_combine adds everything in a fixed order, accumulates
+// the state and mass gradients and forms the next stage's weight.
+// All arithmetic is FP64 and every reduction has a fixed order.
+#include "sol_internal.h"
+
+namespace sol {
+namespace {
+
+__device__ inline int sink_end(const VjpArgs &a, int i) { return i < a.n_massive ? a.ea : a.eb; }
+
+// Sink part of G_r for sinks [1 + 64*blockIdx.x, ...) over the sources of
+// chunk blockIdx.y: psink[(chunk*3 + c)*NP + i] = sum_j gm_j (T(d_ij) u_i)_c.
+__global__ __launch_bounds__(64) void k_vjp_sink(VjpArgs a)
+{
+    const int np = a.np;
+    const int first = 1 + (int)blockIdx.x * kWave;
+    const int i = first + (int)threadIdx.x;
+    const bool live = i < a.n;
+    const int ic = live ? i : a.n - 1;
+    const double xi = a.Y[ic], yi = a.Y[np + ic], zi = a.Y[2 * np + ic];
+    const double ux = a.W[3 * np + ic], uy = a.W[4 * np + ic], uz = a.W[5 * np + ic];
+    const int end_i = sink_end(a, i);
+    const int last = min(first + kWave - 1, a.n - 1);
+    const int tile_end = max(sink_end(a, first), sink_end(a, last));
+    const int js = 1 + (int)blockIdx.y * a.ch_s;
+    const int je = min(min(js + a.ch_s, a.jmax), tile_end);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll 2
+    for (int j = js; j < je; ++j) {
+        const double dx = a.Y[j] - xi, dy = a.Y[np + j] - yi, dz = a.Y[2 * np + j] - zi;
+        const double g = a.gm[j];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const bool ok = live && j < end_i && j != i;
+        const double rinv = ok ? 1.0 / sqrt(r2) : 0.0;
+        const double rinv2 = rinv * rinv;
+        const double c1 = g * rinv * rinv2;
+        const double s = dx * ux + dy * uy + dz * uz;
+        const double c2 = 3.0 * c1 * rinv2 * s;
+        ax += c1 * ux - c2 * dx;
+        ay += c1 * uy - c2 * dy;
+        az += c1 * uz - c2 * dz;
+    }
+    if (live) {
+        double *p = a.psink + (size_t)blockIdx.y * 3 * np;
+        p[i] = ax;
+        p[np + i] = ay;
+        p[2 * np + i] = az;
+    }
+}
+
+// Source sums for sources [1 + 64*blockIdx.x, ...) over the sinks of chunk
+// blockIdx.y: psrc[(chunk*7 + c)*NP + k] = (A_k, B_k, U_k)_c.
+__global__ __launch_bounds__(64) void k_vjp_src(VjpArgs a)
+{
+    const int np = a.np;
+    const int k = 1 + (int)blockIdx.x * kWave + (int)threadIdx.x;
+    const bool live = k < a.jmax;
+    const int kc = live ? k : 1;
+    const double xk = a.Y[kc], yk = a.Y[np + kc], zk = a.Y[2 * np + kc];
+    const int is = 1 + (int)blockIdx.y * a.ch_b;
+    const int ie = min(is + a.ch_b, a.n);
+    double Ax = 0.0, Ay = 0.0, Az = 0.0, B = 0.0, Ux = 0.0, Uy = 0.0, Uz = 0.0;
+#pragma unroll 2
+    for (int i = is; i < ie; ++i) {
+        const bool ok = live && k < sink_end(a, i) && i != k;
+        const double ux = a.W[3 * np + i], uy = a.W[4 * np + i], uz = a.W[5 * np + i];
+        const double dx = xk - a.Y[i], dy = yk - a.Y[np + i], dz = zk - a.Y[2 * np + i];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double rinv = ok ? 1.0 / sqrt(r2) : 0.0;
+        const double rinv2 = rinv * rinv;
+        const double rinv3 = rinv * rinv2;
+        const double s = dx * ux + dy * uy + dz * uz;
+        const double c2 = 3.0 * rinv3 * rinv2 * s;
+        Ax += rinv3 * ux - c2 * dx;
+        Ay += rinv3 * uy - c2 * dy;
+        Az += rinv3 * uz - c2 * dz;
+        B += s * rinv3;
+        const double m = ok ? 1.0 : 0.0;
+        Ux += m * ux;
+        Uy += m * uy;
+        Uz += m * uz;
+    }
+    if (live) {
+        double *p = a.psrc + (size_t)blockIdx.y * 7 * np;
+        p[k] = Ax;
+        p[np + k] = Ay;
+        p[2 * np + k] = Az;
+        p[3 * np + k] = B;
+        p[4 * np + k] = Ux;
+        p[5 * np + k] = Uy;
+        p[6 * np + k] = Uz;
+    }
+}
+
+// Per body: G = VJP(W) from the partials, ybar += G, wnext = alpha*lam +
+// beta*G (when wnext is set), mass gradients accumulated (dL/dgm).
+__global__ __launch_bounds__(kBlock) void k_vjp_combine(VjpArgs a)
+{
+    __shared__ double red[kBlock / kWave];
+    const int np = a.np;
+    const int k = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+    double G[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double m0 = 0.0;
+    if (k >= 1 && k < a.n) {
+        const double x = a.Y[k], y = a.Y[np + k], z = a.Y[2 * np + k];
+        const double ux = a.W[3 * np + k], uy = a.W[4 * np + k], uz = a.W[5 * np + k];
+        const double r2 = x * x + y * y + z * z;
+        const double rinv = 1.0 / sqrt(r2);
+        const double rinv2 = rinv * rinv;
+        const double rinv3 = rinv * rinv2;
+        const double ru = x * ux + y * uy + z * uz;
+        const double mu = a.gm[0] + a.gm[k];
+        const double c = 3.0 * rinv3 * rinv2 * ru;
+        double gx = -mu * (rinv3 * ux - c * x);
+        double gy = -mu * (rinv3 * uy - c * y);
+        double gz = -mu * (rinv3 * uz - c * z);
+        for (int ch = 0; ch < a.nch_s; ++ch) {
+            const double *p = a.psink + (size_t)ch * 3 * np;
+            gx -= p[k];
+            gy -= p[np + k];
+            gz -= p[2 * np + k];
+        }
+        double mg = -rinv3 * ru;
+        m0 = -rinv3 * ru;
+        if (k < a.jmax) {
+            double Ax = 0.0, Ay = 0.0, Az = 0.0, B = 0.0, Ux = 0.0, Uy = 0.0, Uz = 0.0;
+            for (int ch = 0; ch < a.nch_b; ++ch) {
+                const double *p = a.psrc + (size_t)ch * 7 * np;
+                Ax += p[k]; Ay += p[np + k]; Az += p[2 * np + k];
+                B += p[3 * np + k];
+                Ux += p[4 * np + k]; Uy += p[5 * np + k]; Uz += p[6 * np + k];
+            }
+            const double rU = x * Ux + y * Uy + z * Uz;
+            const double cU = 3.0 * rinv3 * rinv2 * rU;
+            const double g = a.gm[k];
+            gx += g * (Ax - (rinv3 * Ux - cU * x));
+            gy += g * (Ay - (rinv3 * Uy - cU * y));
+            gz += g * (Az - (rinv3 * Uz - cU * z));
+            mg += B - rinv3 * rU;
+        }
+        a.mbar[k] += mg;
+        G[0] = gx; G[1] = gy; G[2] = gz;
+        G[3] = a.W[k]; G[4] = a.W[np + k]; G[5] = a.W[2 * np + k];
+    }
+    if (k < np) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const size_t o = (size_t)c * np + k;
+            a.ybar[o] += G[c];
+            if (a.wnext) a.wnext[o] = a.alpha * a.lam[o] + a.beta * G[c];
+        }
+    }
+    // block sum of the central-mass term, fixed order
+    double s = m0;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+        a.m0blk[blockIdx.x] += t;
+    }
+}
+
+// out = y + h*(a*k) over 6 x NP, a stage state of RungeKutta4::Step
+// (Solaris/RungeKutta4.cpp:100-116)
+__global__ __launch_bounds__(kBlock) void k_adj_stage(size_t len, const double *__restrict__ y,
+                                                      const double *__restrict__ k, double a, double h,
+                                                      double *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) out[i] = y[i] + h * (a * k[i]);
+}
+
+// out = y + h*(b1 k1 + b2 k2 + b3 k3 + b4 k4) (Solaris/RungeKutta4.cpp:118-121)
+__global__ __launch_bounds__(kBlock) void k_adj_final(size_t len, const double *__restrict__ y,
+                                                      const double *__restrict__ kb, size_t stride, double h,
+                                                      double *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) {
+        const double b1 = 1.0 / 6.0, b2 = 1.0 / 3.0, b3 = 1.0 / 3.0, b4 = 1.0 / 6.0;
+        out[i] = y[i] + h * (b1 * kb[i] + b2 * kb[stride + i] + b3 * kb[2 * stride + i] + b4 * kb[3 * stride + i]);
+    }
+}
+
+// ybar = lam, w = c*lam
+__global__ __launch_bounds__(kBlock) void k_adj_init(size_t len, const double *__restrict__ lam,
+                                                     double *__restrict__ ybar, double *__restrict__ w, double c)
+{
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < len) {
+        const double l = lam[i];
+        ybar[i] = l;
+        w[i] = c * l;
+    }
+}
+
+// dL/dm from dL/dgm: out[k] = k^2 mbar[k] (k >= 1), out[0] = k^2 sum(m0blk)
+__global__ __launch_bounds__(kBlock) void k_adj_mass(int n, const double *__restrict__ mbar,
+                                                     const double *__restrict__ m0blk, int nblk,
+                                                     double *__restrict__ out)
+{
+    const int k = (int)blockIdx.x * kBlock + (int)threadIdx.x;
+    if (k == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += m0blk[b];
+        out[0] = kGauss2 * s;
+    } else if (k < n) {
+        out[k] = kGauss2 * mbar[k];
+    }
+}
+
+unsigned blocks_for(size_t len) { return (unsigned)((len + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+void launch_vjp(hipStream_t st, const VjpArgs &a)
+{
+    if (a.tiles_s > 0 && a.nch_s > 0)
+        hipLaunchKernelGGL(k_vjp_sink, dim3(a.tiles_s, a.nch_s), dim3(kWave), 0, st, a);
+    if (a.tiles_b > 0 && a.nch_b > 0)
+        hipLaunchKernelGGL(k_vjp_src, dim3(a.tiles_b, a.nch_b), dim3(kWave), 0, st, a);
+    hipLaunchKernelGGL(k_vjp_combine, dim3(a.np / kBlock), dim3(kBlock), 0, st, a);
+}
+
+void launch_adj_stage(hipStream_t st, size_t len, const double *y, const double *k, double a, double h,
+                      double *out)
+{
+    hipLaunchKernelGGL(k_adj_stage, dim3(blocks_for(len)), dim3(kBlock), 0, st, len, y, k, a, h, out);
+}
+
+void launch_adj_final(hipStream_t st, size_t len, const double *y, const double *kb, size_t stride, double h,
+                      double *out)
+{
+    hipLaunchKernelGGL(k_adj_final, dim3(blocks_for(len)), dim3(kBlock), 0, st, len, y, kb, stride, h, out);
+}
+
+void launch_adj_init(hipStream_t st, size_t len, const double *lam, double *ybar, double *w, double c)
+{
+    hipLaunchKernelGGL(k_adj_init, dim3(blocks_for(len)), dim3(kBlock), 0, st, len, lam, ybar, w, c);
+}
+
+void launch_adj_mass(hipStream_t st, int n, const double *mbar, const double *m0blk, int nblk, double *out)
+{
+    hipLaunchKernelGGL(k_adj_mass, dim3(blocks_for((size_t)n)), dim3(kBlock), 0, st, n, mbar, m0blk, nblk, out);
+}
+
+}  // namespace sol

@@ -15,8 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "sol_internal.h"
-#include "resident.h"
+#include "context.h"
 
 using namespace sol;
 
@@ -99,101 +98,19 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
-struct sol_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    sol_nbodies nb{};
-    int n = 0, np = 0, nblk = 0, n_massive = 0;
-    int barycentric = 0;
-    int precision = SOL_PRECISION_FP64;
-    double softening = 0.0;
-    int end_massive = -1, end_other = -1;
-    bool has_gas = false, gas_dead = false;
-    sol_gas_component gas{};
-    GasDev gasdev{};
-    int eval_drag = 1, eval_mig1 = 1, eval_mig2 = 1;
-    int cu_count = 256;
-    int waves_per_simd = 8;
-    int chunk = 256;
-    PairPlan plan{};
-    double interactions_per_eval = 0.0;
-    std::string err;
-
-    // device buffers
-    double *mass = nullptr, *gm = nullptr, *radius = nullptr, *density = nullptr, *cd = nullptr;
-    double *gs = nullptr, *ge = nullptr, *stop = nullptr;
-    int32_t *type = nullptr, *migt = nullptr;
-    double *y0 = nullptr, *y = nullptr, *acc0 = nullptr, *yscale = nullptr, *error = nullptr;
-    double *fst = nullptr, *cin = nullptr, *cout = nullptr, *aos = nullptr;
-    double4 *J = nullptr;
-    float4 *J32 = nullptr;     // allocated when the mixed form is selected
-    double *part = nullptr, *nnr2 = nullptr;
-    int32_t *nnidx = nullptr;
-    double *ind = nullptr;
-    double *rm3 = nullptr, *nnd = nullptr;
-    int32_t *nni = nullptr;
-    double *migc = nullptr, *dragc = nullptr;
-    double *errblk = nullptr, *scal = nullptr, *coef = nullptr, *integ = nullptr;
-    double *h_scal = nullptr;   // pinned: [0] h, [1] t, [2] error max
-    size_t part_slots = 0;
-    std::vector<void *> allocs;
-    // workgroup-resident drivers (resident.hip) for small gas-free systems
-    ResSystem *res_d = nullptr, *res_h = nullptr;
-    bool resident_disabled = false;
-    size_t lds_limit = 65536;
-    // profiling (sol_profile)
-    bool profiling = false;
-    std::vector<hipEvent_t> ev_pool;
-    struct Timed { hipEvent_t a, b; int kind; double inter; };
-    std::vector<Timed> timed;
-    size_t ev_used = 0;
-    sol_kernel_profile prof{};
-    double pair_inter_per_launch = 0.0;
-
-    int fail(int code, const char *fmt, ...)
-    {
-        char buf[512];
-        va_list ap;
-        va_start(ap, fmt);
-        vsnprintf(buf, sizeof(buf), fmt, ap);
-        va_end(ap);
-        err = buf;
-        return code;
-    }
-};
-
-#define HIPCHECK(ctx, call)                                                                    \
-    do {                                                                                       \
-        hipError_t e_ = (call);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return (ctx)->fail(SOL_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,                \
-                               hipGetErrorString(e_), __FILE__, __LINE__);                     \
-    } while (0)
-
-namespace {
-
-template <typename T>
-int dalloc(sol_context *c, T **p, size_t count)
-{
-    void *q = nullptr;
-    HIPCHECK(c, hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
-    HIPCHECK(c, hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(T)));
-    c->allocs.push_back(q);
-    *p = static_cast<T *>(q);
-    return SOL_OK;
-}
-
-int n_pl_spl(const sol_nbodies &nb) { return nb.super_planetesimal + nb.planetesimal; }
-
-bool mixed_on(const sol_context *c) { return c->precision == SOL_PRECISION_MIXED && c->J32 != nullptr; }
-
 // Source range ends of the two astrocentric sink classes
 // (Solaris/Acceleration.cpp:285-289).
-void source_ends(const sol_context *c, int &ea, int &eb)
+void sol::source_ends(const sol_context *c, int &ea, int &eb)
 {
     ea = c->end_massive >= 0 ? c->end_massive : c->n_massive + c->nb.super_planetesimal;
     eb = c->end_other >= 0 ? c->end_other : c->n_massive;
 }
+
+namespace {
+
+int n_pl_spl(const sol_nbodies &nb) { return nb.super_planetesimal + nb.planetesimal; }
+
+bool mixed_on(const sol_context *c) { return c->precision == SOL_PRECISION_MIXED && c->J32 != nullptr; }
 
 // Builds the pair-sum work plan for the context's sink classes.
 void build_plan(sol_context *c)
@@ -630,6 +547,11 @@ int check_ctx(sol_context *c)
 
 }  // namespace
 
+int sol::eval_force(sol_context *c, const double *st, double t, double *out)
+{
+    return evaluate_at(c, st, t, 0.0, out, EPI_COMPUTE, INTEG_RK4, false, false);
+}
+
 extern "C" {
 
 const char *sol_version(void) { return "solaris_amd 0.1 (gfx950)"; }
@@ -724,6 +646,7 @@ int sol_destroy(sol_context *c)
     if (!c) return SOL_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    adjoint_release(c);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->part) (void)hipFree(c->part);
     if (c->nnr2) (void)hipFree(c->nnr2);
@@ -739,6 +662,7 @@ int sol_destroy(sol_context *c)
 int sol_set_bodies(sol_context *c, const sol_body_params *p)
 {
     if (check_ctx(c) || !p) return SOL_ERR_ARGUMENT;
+    adjoint_invalidate(c);
     HIPCHECK(c, hipSetDevice(c->device));
     const size_t n = c->n;
     std::vector<double> zero(n, 0.0), gm(n);
@@ -769,6 +693,7 @@ int sol_set_bodies(sol_context *c, const sol_body_params *p)
 int sol_set_nebula(sol_context *c, const sol_gas_component *g)
 {
     if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    adjoint_invalidate(c);
     HIPCHECK(c, hipSetDevice(c->device));
     c->has_gas = g != nullptr;
     c->gas_dead = false;
@@ -798,6 +723,7 @@ int sol_get_evaluate(const sol_context *c, int32_t *d, int32_t *m1, int32_t *m2)
 int sol_set_precision(sol_context *c, int32_t precision, double softening)
 {
     if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    adjoint_invalidate(c);
     if (precision != SOL_PRECISION_FP64 && precision != SOL_PRECISION_MIXED)
         return c->fail(SOL_ERR_ARGUMENT, "unknown precision %d", precision);
     if (!(softening >= 0.0)) return c->fail(SOL_ERR_ARGUMENT, "softening must be >= 0");
@@ -814,6 +740,7 @@ int sol_set_precision(sol_context *c, int32_t precision, double softening)
 int sol_set_source_ranges(sol_context *c, int32_t a, int32_t b)
 {
     if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    adjoint_invalidate(c);
     if (a > c->n || b > c->n) return c->fail(SOL_ERR_ARGUMENT, "source range beyond total");
     c->end_massive = a;
     c->end_other = b;

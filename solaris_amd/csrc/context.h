@@ -1,0 +1,116 @@
+// context.h -- the context object behind the sol_context handle of
+// include/solaris_amd.h, shared by context.cpp (forces, drivers) and
+// adjoint.cpp (the RK4 adjoint).
+#pragma once
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "sol_internal.h"
+#include "resident.h"
+
+namespace sol { struct AdjointState; }
+
+struct sol_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    sol_nbodies nb{};
+    int n = 0, np = 0, nblk = 0, n_massive = 0;
+    int barycentric = 0;
+    int precision = SOL_PRECISION_FP64;
+    double softening = 0.0;
+    int end_massive = -1, end_other = -1;
+    bool has_gas = false, gas_dead = false;
+    sol_gas_component gas{};
+    sol::GasDev gasdev{};
+    int eval_drag = 1, eval_mig1 = 1, eval_mig2 = 1;
+    int cu_count = 256;
+    int waves_per_simd = 8;
+    int chunk = 256;
+    sol::PairPlan plan{};
+    double interactions_per_eval = 0.0;
+    std::string err;
+
+    // device buffers
+    double *mass = nullptr, *gm = nullptr, *radius = nullptr, *density = nullptr, *cd = nullptr;
+    double *gs = nullptr, *ge = nullptr, *stop = nullptr;
+    int32_t *type = nullptr, *migt = nullptr;
+    double *y0 = nullptr, *y = nullptr, *acc0 = nullptr, *yscale = nullptr, *error = nullptr;
+    double *fst = nullptr, *cin = nullptr, *cout = nullptr, *aos = nullptr;
+    double4 *J = nullptr;
+    float4 *J32 = nullptr;     // allocated when the mixed form is selected
+    double *part = nullptr, *nnr2 = nullptr;
+    int32_t *nnidx = nullptr;
+    double *ind = nullptr;
+    double *rm3 = nullptr, *nnd = nullptr;
+    int32_t *nni = nullptr;
+    double *migc = nullptr, *dragc = nullptr;
+    double *errblk = nullptr, *scal = nullptr, *coef = nullptr, *integ = nullptr;
+    double *h_scal = nullptr;   // pinned: [0] h, [1] t, [2] error max
+    size_t part_slots = 0;
+    std::vector<void *> allocs;
+    // workgroup-resident drivers (resident.hip) for small gas-free systems
+    sol::ResSystem *res_d = nullptr, *res_h = nullptr;
+    bool resident_disabled = false;
+    size_t lds_limit = 65536;
+    // profiling (sol_profile)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Timed { hipEvent_t a, b; int kind; double inter; };
+    std::vector<Timed> timed;
+    size_t ev_used = 0;
+    sol_kernel_profile prof{};
+    double pair_inter_per_launch = 0.0;
+    // recorded forward pass of the RK4 adjoint (adjoint.cpp)
+    sol::AdjointState *adj = nullptr;
+
+    int fail(int code, const char *fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPCHECK(ctx, call)                                                                    \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return (ctx)->fail(SOL_ERR_DEVICE, "%s failed: %s (%s:%d)", #call,                \
+                               hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+namespace sol {
+
+// Device allocation owned by the context (freed by sol_destroy), zeroed.
+template <typename T>
+int dalloc(sol_context *c, T **p, size_t count)
+{
+    void *q = nullptr;
+    HIPCHECK(c, hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    HIPCHECK(c, hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(T)));
+    c->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return SOL_OK;
+}
+
+// Source range ends of the two astrocentric sink classes.
+void source_ends(const sol_context *c, int &ea, int &eb);
+
+// One FP64 force evaluation (Acceleration::Compute) of the device state
+// `st` (6 x NP) at time t into `out` (6 x NP), on the context's stream.
+int eval_force(sol_context *c, const double *st, double t, double *out);
+
+// Frees the adjoint's recorded state (adjoint.cpp).
+void adjoint_release(sol_context *c);
+// Marks the recorded forward pass stale (parameters or force model changed).
+void adjoint_invalidate(sol_context *c);
+
+}  // namespace sol

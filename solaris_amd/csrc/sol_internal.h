@@ -96,6 +96,26 @@ struct EpiArgs {
     const double *coef;       // integrator coefficient tables (context.cpp)
 };
 
+// Arguments of one force vector-Jacobian product (adjoint.hip).  Y is the
+// stage state and W the weight (both 6 x NP); the sink pass runs
+// tiles_s x nch_s waves (sinks [1, n) in tiles of 64, sources [1, jmax) in
+// chunks of ch_s), the source pass tiles_b x nch_b (sources [1, jmax) in
+// tiles of 64, sinks [1, n) in chunks of ch_b).
+struct VjpArgs {
+    int32_t n, np, n_massive, ea;
+    int32_t eb, jmax, tiles_s, nch_s;
+    int32_t ch_s, tiles_b, nch_b, ch_b;
+    const double *Y, *W, *gm;
+    double *psink;            // [nch_s][3][NP]
+    double *psrc;             // [nch_b][7][NP]
+    const double *lam;        // adjoint of the step's end state
+    double *ybar;             // adjoint of the step's start state, accumulated
+    double *wnext;            // next stage weight alpha*lam + beta*G (or null)
+    double alpha, beta;
+    double *mbar;             // dL/d(k^2 m), accumulated per body
+    double *m0blk;            // central-mass term, accumulated per combine block
+};
+
 // Integrator coefficient tables packed at the offsets the kernels read
 // (275 doubles).
 void fill_coefficients(std::vector<double> &c);
@@ -119,5 +139,13 @@ void launch_soa_to_aos(hipStream_t st, int n, int np, const double *soa, double 
 void launch_flush_small(hipStream_t st, int n, int np, double *a, double *b);
 void launch_integrals(hipStream_t st, int n, int np, const double *y0, const double *mass,
                       double *per_body /* 11 x NP */);
+// adjoint.hip
+void launch_vjp(hipStream_t st, const VjpArgs &a);
+void launch_adj_stage(hipStream_t st, size_t len, const double *y, const double *k, double a, double h,
+                      double *out);
+void launch_adj_final(hipStream_t st, size_t len, const double *y, const double *kb, size_t stride, double h,
+                      double *out);
+void launch_adj_init(hipStream_t st, size_t len, const double *lam, double *ybar, double *w, double c);
+void launch_adj_mass(hipStream_t st, int n, const double *mbar, const double *m0blk, int nblk, double *out);
 
 }  // namespace sol

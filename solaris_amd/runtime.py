@@ -67,6 +67,8 @@ _SIGS = {
     "sol_ensemble_run": (C.c_int, [V, C.c_int32, C.c_double, C.c_int64, C.POINTER(_abi.RunStats)]),
     "sol_ensemble_get_system": (C.c_int, [V, C.c_int32, D, C.POINTER(_abi.TimeLine), I,
                                           C.POINTER(C.c_int64)]),
+    "sol_adjoint_forward": (C.c_int, [V, C.c_double, C.c_double, D, C.c_int64, C.c_int32, D]),
+    "sol_adjoint_backward": (C.c_int, [V, D, D, D]),
 }
 
 
@@ -229,6 +231,28 @@ class Context:
         out = _abi.KernelProfile()
         self._check(self.lib.sol_profile(self.h, int(enable), C.byref(out)))
         return out
+
+    def adjoint_forward(self, t0, h, nsteps, checkpoint_every=50, h_seq=None):
+        """Records nsteps fixed RK4 steps from the device Y0 (sol_adjoint_forward);
+        returns the final state (n x 6)."""
+        out = np.zeros((self.n, 6))
+        hp = None
+        if h_seq is not None:
+            h_seq = np.ascontiguousarray(h_seq, dtype=np.float64)
+            if h_seq.size != nsteps:
+                raise ValueError("h_seq must hold nsteps step sizes")
+            hp = _dp(h_seq)
+        self._check(self.lib.sol_adjoint_forward(self.h, float(t0), float(h), hp, int(nsteps),
+                                                 int(checkpoint_every), _dp(out)))
+        return out
+
+    def adjoint_backward(self, grad_y_final):
+        """dL/dy0 (n x 6) and dL/dm (n) from dL/dy_final (sol_adjoint_backward)."""
+        g = np.ascontiguousarray(grad_y_final, dtype=np.float64).reshape(self.n, 6)
+        gy0 = np.zeros((self.n, 6))
+        gm = np.zeros(self.n)
+        self._check(self.lib.sol_adjoint_backward(self.h, _dp(g), _dp(gy0), _dp(gm)))
+        return gy0, gm
 
     def last_error(self):
         return self.lib.sol_last_error(self.h).decode()

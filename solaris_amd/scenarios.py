@@ -162,7 +162,7 @@ class Scenario:
     h_next: float = 0.0
     length: float = 0.0
     softening: float = 0.0
-    targets: Optional[np.ndarray] = None
+    target_offsets: Optional[np.ndarray] = None   # (N, 3), adjoint loss targets
 
     @property
     def n(self) -> int:
@@ -336,8 +336,9 @@ def make_c5(n_tp: int = 1024, n_pl: int = 8, seed: int = SEEDS["c5"]) -> Scenari
     s = _assemble("c5", _counts(central_body=1, proto_planet=n_pl, test_particle=n_tp),
                   [_star(), planets, tps], integrator=RUNGE_KUTTA4, h_next=1.0,
                   length=2000.0)
-    s.targets = s.y0[:, :3] + rng.normal(s.n * 3, 1.0e-3).reshape(s.n, 3)
-    s.targets[0] = 0.0
+    # loss targets r_i* = r_i(T) of the unperturbed run + these offsets
+    s.target_offsets = rng.normal(s.n * 3, 1.0e-3).reshape(s.n, 3)
+    s.target_offsets[0] = 0.0
     return s
 
 

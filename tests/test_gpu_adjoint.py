@@ -1,0 +1,204 @@
+"""GPU adjoint of fixed-step RK4 (sol_adjoint_forward / sol_adjoint_backward)
+on the C5 workload: central body, 8 protoplanets and 1024 test particles,
+2000 RK4 steps of 1 day, checkpoint every 50 steps, loss
+L = sum_i |r_i(T) - r_i*|^2 with r_i* = r_i(T) of the unperturbed run plus
+seeded N(0, 1e-3 AU) offsets.
+
+The workload has close encounters (test particle 444 passes 0.0047 AU from
+protoplanet 6, particle 451 0.006 AU from protoplanet 7): a 1e-15 relative
+change of y0 moves their final positions by ~3e-9 AU.  Hence:
+* the recorded forward pass is compared with the oracle's RK4 within
+  1e-9 AU over the first 100 steps and within 1e-6 AU after 2000;
+* the gradient is compared with the oracle's forward-mode dual-number
+  derivative (cr_rk4_tangent) along random directions in the initial
+  state and the masses, contracted with the same dL/dy(T), to 1e-8 of the
+  sum of the absolute terms;
+* single entries are compared with fourth-order central finite
+  differences of the device forward pass (relative 1e-6) where the finite
+  difference itself converges: test-particle states and a protoplanet mass
+  (for the central mass and the encounter planets it wanders by 1e-6..1e-4
+  with the difference step);
+* the checkpoint interval does not change the gradient (the recomputed
+  segments replay the recorded steps bit for bit), and a non-uniform step
+  sequence is replayed as recorded;
+* unsupported force models are refused.
+"""
+import numpy as np
+import pytest
+
+from oracle.cpu_ref import Oracle
+from solaris_amd import runtime, scenarios as S
+
+from adjoint_common import central_difference, loss_and_grad, oracle_rk4, with_masses
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 2000
+H = 1.0
+K = 50
+
+
+@pytest.fixture(scope="module")
+def c5():
+    return S.make_c5()
+
+
+@pytest.fixture(scope="module")
+def targets(c5):
+    ctx = runtime.Context(c5)
+    ctx.set_state(c5.y0)
+    yT = ctx.adjoint_forward(0.0, H, STEPS, K)
+    ctx.close()
+    return yT[:, :3] + c5.target_offsets
+
+
+@pytest.fixture(scope="module")
+def recorded(c5, targets):
+    ctx = runtime.Context(c5)
+    ctx.set_state(c5.y0)
+    yT = ctx.adjoint_forward(0.0, H, STEPS, K)
+    L, g = loss_and_grad(yT, targets)
+    gy0, gm = ctx.adjoint_backward(g)
+    yield ctx, yT, L, g, gy0, gm
+    ctx.close()
+
+
+def test_forward_matches_oracle_rk4(c5, recorded):
+    _, yT, *_ = recorded
+    ctx = runtime.Context(c5)
+    ctx.set_state(c5.y0)
+    y100 = ctx.adjoint_forward(0.0, H, 100, K)
+    ctx.close()
+    ref100 = oracle_rk4(c5, c5.y0, H, 100)
+    err100 = np.max(np.abs(y100[:, :3] - ref100[:, :3]))
+    ref = oracle_rk4(c5, c5.y0, H, STEPS)
+    err = np.max(np.abs(yT[:, :3] - ref[:, :3]))
+    print(f"C5 forward: max |dr| = {err100:.3e} AU after 100 steps, {err:.3e} AU after {STEPS}")
+    assert err100 <= 1e-9
+    assert err <= 1e-6
+
+
+def _directional(gy0, gm, dy0, dm):
+    terms = np.concatenate([(gy0 * dy0).ravel(), gm * dm])
+    return float(np.sum(terms)), float(np.sum(np.abs(terms)))
+
+
+@pytest.mark.parametrize("kind", ["state", "mass", "both"])
+def test_gradient_matches_dual_numbers(c5, recorded, kind):
+    _, _, _, g, gy0, gm = recorded
+    rng = np.random.default_rng({"state": 1, "mass": 2, "both": 3}[kind])
+    dy0 = np.zeros_like(c5.y0)
+    dm = np.zeros(c5.n)
+    if kind in ("state", "both"):
+        dy0[1:] = rng.normal(size=(c5.n - 1, 6))
+        dy0[:, 3:] *= 0.01
+    if kind in ("mass", "both"):
+        dm[:9] = rng.uniform(0.5, 1.5, 9) * c5.mass[:9]
+    o = Oracle(c5)
+    _, dyT = o.rk4_tangent(c5.y0, H, STEPS, dy0=dy0, dm=dm)
+    dual = float(np.sum(g * dyT))
+    adj, scale = _directional(gy0, gm, dy0, dm)
+    print(f"{kind}: adjoint {adj:.15e} dual {dual:.15e} rel {abs(adj - dual) / abs(dual):.2e}")
+    assert abs(adj - dual) <= 1e-8 * scale
+
+
+def _loss_at(ctx, c5, targets, y0=None, mass=None):
+    if mass is not None:
+        ctx.set_bodies(with_masses(c5, mass))
+    ctx.set_state(c5.y0 if y0 is None else y0)
+    yT = ctx.adjoint_forward(0.0, H, STEPS, K)
+    if mass is not None:
+        ctx.set_bodies(c5)
+    return loss_and_grad(yT, targets)[0]
+
+
+@pytest.mark.parametrize("param", ["mass4", "x9", "y500", "z700", "vx9", "vy500", "vz700"])
+def test_gradient_matches_finite_differences(c5, targets, param):
+    ctx = runtime.Context(c5)
+    ctx.set_state(c5.y0)
+    yT = ctx.adjoint_forward(0.0, H, STEPS, K)
+    _, g = loss_and_grad(yT, targets)
+    gy0, gm = ctx.adjoint_backward(g)
+    if param.startswith("mass"):
+        b = int(param[4:])
+        delta = 0.01 * c5.mass[b]
+
+        def f(e):
+            m = c5.mass.copy()
+            m[b] += e
+            return _loss_at(ctx, c5, targets, mass=m)
+
+        ad = gm[b]
+    else:
+        comp = {"x": 0, "y": 1, "z": 2, "vx": 3, "vy": 4, "vz": 5}[param.rstrip("0123456789")]
+        b = int(param.lstrip("xyzv"))
+        delta = 1e-5 if comp < 3 else 1e-7
+
+        def f(e):
+            y0 = c5.y0.copy()
+            y0[b, comp] += e
+            return _loss_at(ctx, c5, targets, y0=y0)
+
+        ad = gy0[b, comp]
+    fd = central_difference(f, delta)
+    ctx.close()
+    print(f"{param}: adjoint {ad:.12e} fd {fd:.12e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) <= 1e-6 * abs(fd)
+
+
+def test_checkpoint_interval_does_not_change_gradient(c5):
+    steps = 200
+    out = []
+    for k in (50, 1, 7, steps):
+        ctx = runtime.Context(c5)
+        ctx.set_state(c5.y0)
+        yT = ctx.adjoint_forward(0.0, H, steps, k)
+        _, g = loss_and_grad(yT, yT[:, :3] + c5.target_offsets)
+        out.append(ctx.adjoint_backward(g))
+        ctx.close()
+    for gy0, gm in out[1:]:
+        assert np.array_equal(gy0, out[0][0])
+        assert np.array_equal(gm, out[0][1])
+
+
+def test_step_sequence_is_replayed(c5):
+    hs = 0.5 + (np.arange(300) % 4) * 0.25
+    ctx = runtime.Context(c5)
+    ctx.set_state(c5.y0)
+    yT = ctx.adjoint_forward(0.0, 0.0, len(hs), 16, h_seq=hs)
+    ref = oracle_rk4(c5, c5.y0, hs, len(hs))
+    assert np.max(np.abs(yT[:, :3] - ref[:, :3])) <= 1e-6
+    _, g = loss_and_grad(yT, yT[:, :3] + c5.target_offsets)
+    gy0, gm = ctx.adjoint_backward(g)
+    ctx.close()
+    rng = np.random.default_rng(5)
+    dy0 = np.zeros_like(c5.y0)
+    dy0[1:] = rng.normal(size=(c5.n - 1, 6))
+    dy0[:, 3:] *= 0.01
+    dm = np.zeros(c5.n)
+    dm[:9] = c5.mass[:9]
+    _, dyT = Oracle(c5).rk4_tangent(c5.y0, 0.0, len(hs), dy0=dy0, dm=dm, h_seq=hs)
+    dual = float(np.sum(g * dyT))
+    adj, scale = _directional(gy0, gm, dy0, dm)
+    assert abs(adj - dual) <= 1e-8 * scale
+
+
+def test_unsupported_models_are_refused(c5):
+    ctx = runtime.Context(c5, barycentric=True)
+    ctx.set_state(c5.y0)
+    with pytest.raises(runtime.SolarisError) as e:
+        ctx.adjoint_forward(0.0, H, 10, 5)
+    assert e.value.code == 7
+    ctx.close()
+    gas = S.make_c2(n_pl=64).gas
+    ctx = runtime.Context(c5)
+    ctx.set_nebula(gas)
+    ctx.set_state(c5.y0)
+    with pytest.raises(runtime.SolarisError) as e:
+        ctx.adjoint_forward(0.0, H, 10, 5)
+    assert e.value.code == 7
+    ctx.set_nebula(None)
+    with pytest.raises(runtime.SolarisError) as e:
+        ctx.adjoint_backward(np.zeros((c5.n, 6)))
+    assert e.value.code == 2
+    ctx.close()
