@@ -218,7 +218,9 @@ int sol_driver(sol_context *ctx, int32_t integrator, double epsilon,
 /* One Step call from device Y0 and ACCEL at (time, h) into Y and ERROR:
  *   RungeKuttaFehlberg78::Step (Solaris/RungeKuttaFehlberg78.cpp:147-250),
  *   RungeKutta4::Step          (Solaris/RungeKutta4.cpp:58-130),
- *   DormandPrince::Step2       (Solaris/DormandPrince.cpp:246-491). */
+ *   DormandPrince::Step2       (Solaris/DormandPrince.cpp:246-491).
+ * RungeKutta4 leaves ERROR as it was, as the reference leaves
+ * BodyData::error. */
 int sol_step(sol_context *ctx, int32_t integrator, double time, double h);
 /* RungeKuttaFehlberg78::GetErrorMax (Solaris/RungeKuttaFehlberg78.cpp:252-262)
  * over the device ERROR and YSCALE arrays, divided by epsilon; with

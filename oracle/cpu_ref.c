@@ -737,7 +737,7 @@ static int rkf78_driver(cr_system *s, double eps, sol_timeline *tl)
 static int rk4_step(cr_system *s)
 {
     const size_t nv = 6*(size_t)s->n;
-    double *f1 = malloc(nv*8), *f2 = malloc(nv*8), *f3 = malloc(nv*8), *yt = malloc(nv*8);
+    double *f1 = malloc(nv*8), *f2 = malloc(nv*8), *f3 = malloc(nv*8), *yt = calloc(nv, 8);
     const double h = s->h, t = s->time;
     const double *f0 = s->accel;
     for (size_t i = 0; i < nv; i++) yt[i] = s->y0[i] + h*(0.5*f0[i]);

@@ -142,13 +142,25 @@ void launch_fp64(hipStream_t st, const PairPlan &plan, const double4 *J, double 
 // the products), summed in FP32 over one chunk of at most `chunk` sources
 // and folded into FP64 accumulators per chunk, so that the accumulation
 // error grows with the chunk length, not with N.  J32 holds the FP32
-// copies (x, y, z, k^2 m) of the same positions.  The nearest neighbour is
-// chosen on the FP32 squared distance; its distance is recomputed in FP64
-// by the epilogue.
+// copies (x, y, z, k^2 m) of the same positions.
+//
+// Close pairs are evaluated in FP64 from J.  Rounding the absolute
+// positions to FP32 moves d = r_j - r_i by up to ~2^-24 (|r_i| + |r_j|), a
+// relative error of 2^-24 |r_i| / |d| that grows without bound as the pair
+// closes in; so a pair with |d| < near_frac |r_i| (FP32 test, a wave-level
+// branch taken only when some lane has such a pair) takes the FP64 path
+// instead.  With near_frac = 3e-3 the FP32 terms keep |dd|/|d| below
+// ~4e-5, and a close pair's term, which can rival the star's, is exact to
+// FP64 rounding.
+//
+// The nearest neighbour is chosen on the FP32 squared distance; its
+// distance is recomputed in FP64 by the epilogue.
 template <bool NN, bool BC, bool SOFT>
 __global__ __launch_bounds__(256) void k_pairs_mixed(PairPlan plan, const float4 *__restrict__ J32,
+                                                      const double4 *__restrict__ J,
                                                       double *__restrict__ part, double *__restrict__ nnr2,
-                                                      int32_t *__restrict__ nnidx, float eps2)
+                                                      int32_t *__restrict__ nnidx, float eps2, double eps2d,
+                                                      float near_frac2, double c15, double c1875)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -158,7 +170,8 @@ __global__ __launch_bounds__(256) void k_pairs_mixed(PairPlan plan, const float4
 
     int slot = 0, cur = -1;
     int sb = 0, se = 0, i = 0;
-    float xi = 0.f, yi = 0.f, zi = 0.f;
+    float xi = 0.f, yi = 0.f, zi = 0.f, thr2 = 0.f;
+    double xd = 0.0, yd = 0.0, zd = 0.0;
     double ax = 0.0, ay = 0.0, az = 0.0;
     float best = FLT_MAX;
     int bidx = -1;
@@ -173,6 +186,21 @@ __global__ __launch_bounds__(256) void k_pairs_mixed(PairPlan plan, const float4
             nnr2[nb + lane] = (double)best;
             nnidx[nb + lane] = bidx;
         }
+    };
+    // FP64 pair term of source j, added to the FP64 accumulators
+    auto near_term = [&](int j) {
+        const double4 q = J[j];
+        const double dx = q.x - xd, dy = q.y - yd, dz = q.z - zd;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double rs = SOFT ? r2 + eps2d : r2;
+        const double y0 = __builtin_amdgcn_rsq(rs);
+        const double tt = y0 * y0;
+        const double e = __builtin_fma(-rs, tt, 1.0);
+        const double mq = q.w * (tt * y0);
+        const double f = __builtin_fma(mq, e * __builtin_fma(c1875, e, c15), mq);
+        ax = __builtin_fma(f, dx, ax);
+        ay = __builtin_fma(f, dy, ay);
+        az = __builtin_fma(f, dz, az);
     };
 
     for (int64_t it = lo; it < hi; ++it) {
@@ -190,19 +218,26 @@ __global__ __launch_bounds__(256) void k_pairs_mixed(PairPlan plan, const float4
             i = min(sb + lane, se - 1);
             const float4 me = J32[i];
             xi = me.x; yi = me.y; zi = me.z;
+            const double4 md = J[i];
+            xd = md.x; yd = md.y; zd = md.z;
+            thr2 = near_frac2 * __builtin_fmaf(zi, zi, __builtin_fmaf(yi, yi, xi * xi));
         }
         const int jb = sg.src_begin + c * plan.chunk;
         const int je = min(jb + plan.chunk, sg.src_end);
+        const bool has_self = !(jb >= se || sb >= je);
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        if (jb >= se || sb >= je) {
-            // no self pair in this chunk
+        bool lane_near = false;   // this lane met a close pair in this chunk
+        if (!has_self) {
 #pragma unroll 4
             for (int j = jb; j < je; ++j) {
                 const float4 q = J32[j];
                 const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const float y0 = __builtin_amdgcn_rsqf(SOFT ? r2 + eps2 : r2);
-                const float f = (q.w * y0) * (y0 * y0);
+                const bool near = r2 < thr2;
+                lane_near |= near;
+                const float y0m = near ? 0.0f : y0;
+                const float f = (q.w * y0m) * (y0 * y0);
                 fx = __builtin_fmaf(f, dx, fx);
                 fy = __builtin_fmaf(f, dy, fy);
                 fz = __builtin_fmaf(f, dz, fz);
@@ -218,8 +253,10 @@ __global__ __launch_bounds__(256) void k_pairs_mixed(PairPlan plan, const float4
                 const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const float rs = self ? 1.0f : (SOFT ? r2 + eps2 : r2);
                 const float y0 = __builtin_amdgcn_rsqf(rs);
-                const float m = self ? 0.0f : q.w;
-                const float f = (m * y0) * (y0 * y0);
+                const bool near = !self && r2 < thr2;
+                lane_near |= near;
+                const float y0m = (self || near) ? 0.0f : y0;
+                const float f = (q.w * y0m) * (y0 * y0);
                 fx = __builtin_fmaf(f, dx, fx);
                 fy = __builtin_fmaf(f, dy, fy);
                 fz = __builtin_fmaf(f, dz, fz);
@@ -231,17 +268,31 @@ __global__ __launch_bounds__(256) void k_pairs_mixed(PairPlan plan, const float4
         ax += (double)fx;
         ay += (double)fy;
         az += (double)fz;
+        if (__any(lane_near)) {
+            // second pass over the chunk for its close pairs, in FP64 (rare:
+            // taken by the waves whose sinks have a source this close)
+            for (int j = jb; j < je; ++j) {
+                const float4 q = J32[j];
+                const float dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                if (j != i && r2 < thr2) near_term(j);
+            }
+        }
     }
     if (cur >= 0) flush(slot);
 }
 
+// Pairs closer than this fraction of the sink's distance from the origin
+// are evaluated in FP64 by the mixed kernel.
+constexpr float kNearFrac = 3.0e-3f;
+
 template <bool NN, bool BC, bool SOFT>
-void launch_mixed(hipStream_t st, const PairPlan &plan, const float4 *J32, double *part,
-                  double *nnr2, int32_t *nnidx, float eps2)
+void launch_mixed(hipStream_t st, const PairPlan &plan, const float4 *J32, const double4 *J, double *part,
+                  double *nnr2, int32_t *nnidx, double eps2)
 {
     const int blocks = (plan.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
     hipLaunchKernelGGL((k_pairs_mixed<NN, BC, SOFT>), dim3(blocks), dim3(kBlock), 0, st,
-                       plan, J32, part, nnr2, nnidx, eps2);
+                       plan, J32, J, part, nnr2, nnidx, (float)eps2, eps2, kNearFrac * kNearFrac, 1.5, 1.875);
 }
 
 }  // namespace
@@ -253,8 +304,7 @@ void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double
     if (plan.total_items == 0) return;
     const bool soft = eps2 != 0.0;
     if (mixed) {
-        const float e2 = (float)eps2;
-#define SOL_MIXED(NN, BC, SOFT) launch_mixed<NN, BC, SOFT>(st, plan, J32, part, nnr2, nnidx, e2)
+#define SOL_MIXED(NN, BC, SOFT) launch_mixed<NN, BC, SOFT>(st, plan, J32, J, part, nnr2, nnidx, eps2)
         if (nn) {
             if (bc) { if (soft) SOL_MIXED(true, true, true); else SOL_MIXED(true, true, false); }
             else    { if (soft) SOL_MIXED(true, false, true); else SOL_MIXED(true, false, false); }

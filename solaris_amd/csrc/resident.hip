@@ -308,7 +308,6 @@ __global__ __launch_bounds__(1024) void k_resident(ResSystem *systems, ResArrays
                         at(A.y, c, k, i) = cat(A.y0, c, k, i) +
                             h * (B[0] * F(c, 0, k, i) + B[1] * F(c, 1, k, i) + B[2] * F(c, 2, k, i) +
                                  B[3] * F(c, 3, k, i));
-                        at(A.err, c, k, i) = 0.0;
                     }
                 } else {
                     const double *B = coef + kDpB, *Bd = coef + kDpBdh;

@@ -530,7 +530,7 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
                     ratio = fmax(ratio, fabs(e[c] / a.yscale[c * np + i]));
                 }
             } else if (a.integ == INTEG_RK4) {
-                // RungeKutta4.cpp:119-121
+                // RungeKutta4.cpp:119-121; the step leaves BodyData::error as it was
                 const double *B = coef + kRk4B;
                 for (int c = 0; c < 6; ++c) {
                     y[c] = a.y0[c * np + i] + h * (B[0] * fetch_f(a, 0, c, i, s, f) + B[1] * fetch_f(a, 1, c, i, s, f) +
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
             }
             for (int c = 0; c < 6; ++c) {
                 a.y_out[c * np + i] = y[c];
-                a.err_out[c * np + i] = e[c];
+                if (a.integ != INTEG_RK4) a.err_out[c * np + i] = e[c];
             }
         }
         ratio = wave_max(ratio);

@@ -196,15 +196,20 @@ int RunDriver(int integrator, double epsilon, BodyData *bd, Acceleration *acc, T
     acc->evaluateGasDrag = true;
     acc->evaluateTypeIMigration = false;
     acc->evaluateTypeIIMigration = false;
+    // RungeKutta4 never writes BodyData::error and only RungeKuttaFehlberg78
+    // writes yscale (Solaris/RungeKutta4.cpp:58-130, DormandPrince.cpp:471-483)
+    const bool has_error = integrator != SOL_RUNGE_KUTTA4;
+    const bool has_yscale = integrator == SOL_RUNGE_KUTTA_FEHLBERG78;
     if (r) {
         // no swap on failure; y and error hold the last attempt
         sol_get_array(ctx, SOL_ARRAY_Y, bd->y);
-        sol_get_array(ctx, SOL_ARRAY_ERROR, bd->error);
+        if (has_error) sol_get_array(ctx, SOL_ARRAY_ERROR, bd->error);
         return fail("Driver", ctx);
     }
     bd->time = tl->time;
     if (sol_get_array(ctx, SOL_ARRAY_Y0, bd->y) || sol_get_array(ctx, SOL_ARRAY_ACCEL, bd->accel) ||
-        sol_get_array(ctx, SOL_ARRAY_YSCALE, bd->yscale) || sol_get_array(ctx, SOL_ARRAY_ERROR, bd->error))
+        (has_yscale && sol_get_array(ctx, SOL_ARRAY_YSCALE, bd->yscale)) ||
+        (has_error && sol_get_array(ctx, SOL_ARRAY_ERROR, bd->error)))
         return fail("sol_get_array", ctx);
     std::swap(bd->y0, bd->y);
     return 0;
@@ -216,7 +221,8 @@ int RunStep(int integrator, BodyData *bd, Acceleration *acc)
     if (!ctx) return 1;
     if (sol_set_state(ctx, bd->y0) || sol_set_accel(ctx, bd->accel)) return fail("upload", ctx);
     if (sol_step(ctx, integrator, bd->time, bd->h)) return fail("Step", ctx);
-    if (sol_get_array(ctx, SOL_ARRAY_Y, bd->y) || sol_get_array(ctx, SOL_ARRAY_ERROR, bd->error))
+    if (sol_get_array(ctx, SOL_ARRAY_Y, bd->y) ||
+        (integrator != SOL_RUNGE_KUTTA4 && sol_get_array(ctx, SOL_ARRAY_ERROR, bd->error)))
         return fail("sol_get_array", ctx);
     return FetchSideOutputs(acc, ctx);
 }

@@ -112,9 +112,15 @@ def test_dropin_driver_matches_c_abi_run(case):
     assert np.array_equal(got["accel"], ctx.get_array(runtime.ACCEL))
     rm3, nn, _, mig = ctx.side_outputs()
     assert np.array_equal(got["rm3"], rm3) and np.array_equal(got["nn"], nn) and np.array_equal(got["mig"], mig)
-    # the final Compute() of the drop-in against the oracle
+    # the final Compute() of the drop-in against the oracle.  After a Driver
+    # call the migration switches are off, so Compute re-adds the migration
+    # cached at the last call's first evaluation (time - hDid, the previous
+    # state, which the device Y holds after the y0/y swap): prime the
+    # oracle's cache there first.
     o = Oracle(scn)
     o.set_mig_type(got["mig"])
+    o.set_evaluate(1, 1, 1)
+    o.compute(tl.time - tl.hDid, ctx.get_array(runtime.Y))
     o.set_evaluate(1, 0, 0)
     ref = o.compute(got["time"], got["y0"])
     num = np.linalg.norm(got["dydt"][:, 3:] - ref[:, 3:], axis=1)

@@ -21,6 +21,8 @@ import pytest
 from oracle.cpu_ref import Oracle
 from solaris_amd import _abi, runtime, scenarios as S
 
+from invariants import barycentric_invariants
+
 pytestmark = pytest.mark.gpu
 
 POS_TOL_AU = 1e-9
@@ -86,26 +88,6 @@ def test_dormand_prince_c2_replay_50_steps():
     scn = S.make_c2(n_pl=512)
     print("[dp c2 replay]", flush=True)
     replay(scn, S.DORMAND_PRINCE, 1e-10, 50, 5.0)
-
-
-def barycentric_invariants(y, m):
-    """Energy and angular momentum in the barycentric frame computed from
-    astrocentric states (the reference's Integrals use the astrocentric
-    frame, where they are not conserved)."""
-    msk = m > 0
-    M = m.sum()
-    rb = (m[:, None] * y[:, :3]).sum(0) / M
-    vb = (m[:, None] * y[:, 3:]).sum(0) / M
-    R = y[msk, :3] - rb
-    Vv = y[msk, 3:] - vb
-    mm = m[msk]
-    ke = 0.5 * (mm * (Vv ** 2).sum(1)).sum()
-    pe = 0.0
-    for i in range(len(mm)):
-        d = np.linalg.norm(R[i + 1:] - R[i], axis=1)
-        pe -= S.GAUSS2 * (mm[i] * mm[i + 1:] / d).sum()
-    L = (mm[:, None] * np.cross(R, Vv)).sum(0)
-    return ke + pe, L
 
 
 def test_dormand_prince_c5_replay_resident(path):
