@@ -49,6 +49,7 @@ def run_gpu(scn, steps, warmup, precision):
     out = dict(wall_s=wall, steps=int(st.accepted_steps), attempts=int(st.attempts),
                evaluations=int(st.force_evaluations), interactions=st.interactions,
                device_ms=st.device_ms, pair_launches=int(prof.pair_launches), pair_ms=prof.pair_ms,
+               pair_nn_launches=int(prof.pair_nn_launches), pair_nn_ms=prof.pair_nn_ms,
                pair_interactions=prof.pair_interactions, epilogue_ms=prof.epilogue_ms,
                epilogue_launches=int(prof.epilogue_launches),
                interactions_per_eval=ctx.interactions_per_evaluation())
@@ -83,6 +84,102 @@ def cpu_baseline(seconds: float):
                 steps_per_s_extrapolated=rate / (13.0 * 4294967296.0))
 
 
+def cpu_all_cores(seconds: float):
+    """The oracle restatement built with OpenMP over sinks, on every host
+    core the process may use, same sample as cpu_baseline."""
+    from oracle import cpu_ref
+    from solaris_amd import scenarios as S
+    if not cpu_ref.available("omp"):
+        return None
+    scn = S.make_c3(n=8192, softening=0.0)
+    o = cpu_ref.Oracle(scn, lib="omp")
+    ipe = scn.interactions_per_evaluation()
+    n_eval = 0
+    t0 = time.perf_counter()
+    while True:
+        o.compute(0.0, scn.y0)
+        n_eval += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return dict(value=n_eval * ipe / dt, unit="interactions/s", cores=threads, kind="port",
+                cpu_model=model, sample=f"{n_eval} evaluations at N=8,193, {dt:.1f} s wall")
+
+
+def ensemble_throughput(nsys: int, days: float):
+    """C4-style batch: nsys systems (star + 2..256 protoplanets), RKF78 1e-10,
+    each integrated over `days` with the caller loop's clipping."""
+    from solaris_amd import _abi, runtime, scenarios as S
+    scns = [S.make_c4_system(k) for k in range(nsys)]
+    ens = runtime.Ensemble(scns)
+    for k, s in enumerate(scns):
+        ens.set_system(k, s, _abi.timeline(0.0, s.h_next, days))
+    t0 = time.perf_counter()
+    st = ens.run(S.RUNGE_KUTTA_FEHLBERG78, 1e-10)
+    wall = time.perf_counter() - t0
+    done = sum(1 for k in range(nsys) if ens.get_system(k)[2] == 1)
+    ens.close()
+    return dict(systems=nsys, span_days=days, systems_done=done, accepted_steps=int(st.accepted_steps),
+                interactions=st.interactions, wall_s=wall, interactions_per_s=st.interactions / wall,
+                system_steps_per_s=st.accepted_steps / wall, device_ms=st.device_ms)
+
+
+def adjoint_timing():
+    """C5: 2000 RK4 steps of 1 day, checkpoint every 50, forward + backward."""
+    from solaris_amd import runtime, scenarios as S
+    import numpy as np
+    scn = S.make_c5()
+    ctx = runtime.Context(scn)
+    ctx.set_state(scn.y0)
+    ctx.adjoint_forward(0.0, 1.0, 50, 50)          # warm-up
+    ctx.set_state(scn.y0)
+    t0 = time.perf_counter()
+    yT = ctx.adjoint_forward(0.0, 1.0, 2000, 50)
+    t1 = time.perf_counter()
+    g = np.zeros_like(yT)
+    g[1:, :3] = 2.0 * scn.target_offsets[1:]
+    ctx.adjoint_backward(g)
+    t2 = time.perf_counter()
+    ctx.close()
+    ipe = scn.interactions_per_evaluation()
+    return dict(workload="C5: star + 8 protoplanets + 1024 test particles, 2000 RK4 steps, K=50",
+                forward_ms=1e3 * (t1 - t0), backward_ms=1e3 * (t2 - t1),
+                forward_interactions_per_s=4 * 2000 * ipe / (t1 - t0))
+
+
+def traffic_probe(n: int):
+    """Launch sequence profiled by the two `rocprofv3 --pmc` passes
+    (FETCH_SIZE, then WRITE_SIZE) that give the pair kernel's HBM traffic:
+    a plain device-to-device copy of known size (2 GiB read, 2 GiB
+    written), whose counter readings calibrate the counters, then one
+    accepted RKF78 step of the benchmark workload (12 pair launches without
+    and one with the nearest-neighbour search)."""
+    import torch
+    from solaris_amd import _abi, runtime, scenarios as S
+    src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    scn = S.make_c3(n=n, softening=0.0)
+    ctx = runtime.Context(scn)
+    ctx.set_state(scn.y0)
+    tl = _abi.timeline(0.0, scn.h_next, 1.0e12)
+    r, st = ctx.integrate(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl, max_steps=1)
+    if r:
+        raise RuntimeError(ctx.last_error())
+    ctx.close()
+    print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                      "pair_launches": int(st.force_evaluations)}), flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
@@ -103,11 +200,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the ensemble, adjoint and all-core CPU figures")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only the launch sequence for the FETCH_SIZE / WRITE_SIZE counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
               "How the direct sum would shard over GPUs is described in DESIGN.md.", file=sys.stderr)
         sys.exit(2)
+
+    if args.traffic_probe:
+        traffic_probe(args.n)
+        return
 
     from solaris_amd import runtime, scenarios as S
     scn = S.make_c3(n=args.n, softening=0.0)
@@ -138,13 +243,25 @@ def main():
                    "evaluations_per_accepted_step": g["evaluations"] / max(g["steps"], 1)},
         "steps_per_s": g["steps"] / g["wall_s"],
         "attempts": g["attempts"],
-        "roofline": {"bound": "fp64_vector", "kernel": "k_pairs_fp64",
+        "roofline": {"bound": "fp64_vector", "kernel": "k_pairs_fp64<false, false, false>",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(),
                      "avg_launch_ms": pair_ms, "launches": g["pair_launches"],
+                     "nn_variant_avg_launch_ms": g["pair_nn_ms"] / max(g["pair_nn_launches"], 1),
+                     "nn_variant_launches": g["pair_nn_launches"],
                      "pair_interactions_per_launch": pairs_per_launch,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "epilogue_ms_per_launch": g["epilogue_ms"] / max(g["epilogue_launches"], 1),
+        # RKF78 stage combine, error and norm: the reference's unfused dataflow
+        # moves 4704 N bytes per accepted step and 95*48 N per rejected attempt;
+        # here it is fused into the per-stage epilogues.
+        "stage_combine": {
+            "algorithmic_bytes": 48.0 * (args.n + 1) * (98 * g["steps"] + 95 * (g["attempts"] - g["steps"])),
+            "epilogue_ms": g["epilogue_ms"],
+            "effective_GBps": 48.0 * (args.n + 1) * (98 * g["steps"] + 95 * (g["attempts"] - g["steps"]))
+                              / max(g["epilogue_ms"] * 1e-3, 1e-12) / 1e9,
+            "fusion": "partial sums, Kepler/indirect terms, next-stage combine, final state, error and "
+                      "block error maximum in one epilogue kernel per stage"},
     }
     if not args.no_mixed:
         try:
@@ -156,9 +273,20 @@ def main():
                              "pair_ms_per_launch": m["pair_ms"] / max(m["pair_launches"], 1)}
         except runtime.SolarisError as e:
             line["mixed"] = {"value": None, "note": str(e)}
+    if not args.no_extras:
+        for key, fn in (("ensemble", lambda: ensemble_throughput(4096, 10.0)), ("adjoint", adjoint_timing)):
+            try:
+                line[key] = fn()
+            except Exception as e:      # reported, not fatal: the headline figure stands alone
+                line[key] = {"error": str(e)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         line["cpu_baseline"]["speedup_gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+        if not args.no_extras:
+            allc = cpu_all_cores(min(args.cpu_seconds, 6.0))
+            if allc:
+                allc["speedup_gpu_over_cpu"] = value / allc["value"]
+                line["cpu_all_cores"] = allc
     print(json.dumps(line), flush=True)
 
 

@@ -245,11 +245,13 @@ int sol_integrals(sol_context *ctx, double *out16);
 
 /* Kernel timing of the context's own stream. */
 typedef struct sol_kernel_profile {
-    int64_t pair_launches;     /* pair-sum kernel launches timed */
+    int64_t pair_launches;     /* pair-sum launches without the nearest-neighbour search */
     double  pair_ms;           /* their summed duration (HIP events) */
     double  pair_interactions; /* pair terms they evaluated */
     int64_t epilogue_launches;
     double  epilogue_ms;
+    int64_t pair_nn_launches;  /* pair-sum launches that also track the nearest neighbour */
+    double  pair_nn_ms;
 } sol_kernel_profile;
 /* enable = 1 clears the totals and starts bracketing every pair-sum and
  * epilogue launch with HIP events on the context's stream; enable = 0

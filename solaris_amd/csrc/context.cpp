@@ -219,7 +219,7 @@ struct ProfScope {
         hipEvent_t b = prof_event(c);
         if (!b) return;
         (void)hipEventRecord(b, c->stream);
-        c->timed.push_back({a, b, kind, kind == 0 ? c->pair_inter_per_launch : 0.0});
+        c->timed.push_back({a, b, kind, kind != 1 ? c->pair_inter_per_launch : 0.0});
     }
 };
 
@@ -232,6 +232,7 @@ void prof_collect(sol_context *c)
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, t.a, t.b);
         if (t.kind == 0) { c->prof.pair_launches++; c->prof.pair_ms += ms; c->prof.pair_interactions += t.inter; }
+        else if (t.kind == 2) { c->prof.pair_nn_launches++; c->prof.pair_nn_ms += ms; }
         else { c->prof.epilogue_launches++; c->prof.epilogue_ms += ms; }
     }
     c->timed.clear();
@@ -309,7 +310,7 @@ int evaluate_at(sol_context *c, const double *st, double t, double h, double *ou
                              c->stop, c->migt, c->migc, c->gasdev, c->barycentric, t, do1, do2);
     }
     {
-        ProfScope ps(c, 0);
+        ProfScope ps(c, last ? 2 : 0);
         launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
                      mixed_on(c), c->softening * c->softening, c->J32);
     }
@@ -366,7 +367,7 @@ int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double
     for (int s = 1; s <= S; ++s) {
         const bool last = (s == S);
         {
-            ProfScope ps(c, 0);
+            ProfScope ps(c, last ? 2 : 0);
             launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
                          mixed_on(c), c->softening * c->softening, c->J32);
         }
