@@ -50,6 +50,8 @@ def run_gpu(scn, steps, warmup, precision):
                evaluations=int(st.force_evaluations), interactions=st.interactions,
                device_ms=st.device_ms, pair_launches=int(prof.pair_launches), pair_ms=prof.pair_ms,
                pair_nn_launches=int(prof.pair_nn_launches), pair_nn_ms=prof.pair_nn_ms,
+               pair_sym_launches=int(prof.pair_sym_launches), pair_sym_ms=prof.pair_sym_ms,
+               pair_sym_interactions=prof.pair_sym_interactions,
                pair_interactions=prof.pair_interactions, epilogue_ms=prof.epilogue_ms,
                epilogue_launches=int(prof.epilogue_launches),
                interactions_per_eval=ctx.interactions_per_evaluation())
@@ -180,15 +182,19 @@ def traffic_probe(n: int):
                       "pair_launches": int(st.force_evaluations)}), flush=True)
 
 
-def load_traffic():
+def load_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the calibrated counter passes
+    (profiles/traffic.json), or None when they were not taken for it."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
-            return json.load(f).get("pairs_fp64_c3_bytes_per_launch")
-    except Exception:
+            d = json.load(f)
+    except (OSError, ValueError):
         return None
+    entry = d.get("kernels", {}).get(kernel)
+    return entry.get("bytes_per_launch") if entry else None
 
 
 def main():
@@ -221,8 +227,18 @@ def main():
     g = run_gpu(scn, args.steps, args.warmup, runtime.FP64)
     value = g["interactions"] / g["wall_s"]
     ms_per_step = 1000.0 * g["wall_s"] / max(g["steps"], 1)
-    pair_ms = g["pair_ms"] / max(g["pair_launches"], 1)
-    pairs_per_launch = g["pair_interactions"] / max(g["pair_launches"], 1)
+    # the dominant pair kernel: the symmetric sum when it ran, else the
+    # one-directional sum without the nearest-neighbour search
+    if g["pair_sym_launches"] > 0:
+        kernel = "k_pairs_sym_fp64<false>"
+        launches = g["pair_sym_launches"]
+        pair_ms = g["pair_sym_ms"] / launches
+        pairs_per_launch = g["pair_sym_interactions"] / launches
+    else:
+        kernel = "k_pairs_fp64<false, false, false>"
+        launches = g["pair_launches"]
+        pair_ms = g["pair_ms"] / max(launches, 1)
+        pairs_per_launch = g["pair_interactions"] / max(launches, 1)
     achieved = FLOP_PER_INTERACTION * pairs_per_launch / (pair_ms * 1e-3) / 1e12
     line = {
         "metric": METRIC,
@@ -243,10 +259,10 @@ def main():
                    "evaluations_per_accepted_step": g["evaluations"] / max(g["steps"], 1)},
         "steps_per_s": g["steps"] / g["wall_s"],
         "attempts": g["attempts"],
-        "roofline": {"bound": "fp64_vector", "kernel": "k_pairs_fp64<false, false, false>",
+        "roofline": {"bound": "fp64_vector", "kernel": kernel,
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(),
-                     "avg_launch_ms": pair_ms, "launches": g["pair_launches"],
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(kernel),
+                     "avg_launch_ms": pair_ms, "launches": launches,
                      "nn_variant_avg_launch_ms": g["pair_nn_ms"] / max(g["pair_nn_launches"], 1),
                      "nn_variant_launches": g["pair_nn_launches"],
                      "pair_interactions_per_launch": pairs_per_launch,

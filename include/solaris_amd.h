@@ -195,6 +195,14 @@ int sol_get_side_outputs(sol_context *ctx, double *rm3, int32_t *index_nn,
                          double *distance_nn, int32_t *mig_type);
 /* Overwrites the device copy of BodyData::migType. */
 int sol_set_mig_type(sol_context *ctx, const int32_t *mig_type);
+/* enable = 0: sol_compute skips the nearest-neighbour search and leaves the
+ * rm3 / indexOfNN / distanceOfNN side outputs as they were, which lets it use
+ * the symmetric pair sum of the self-interacting block (each pair once for
+ * both bodies).  1 (the default) writes them as the reference does.  The
+ * drivers are not affected: they track nearest neighbours on the last
+ * evaluation of each attempt only, as the reference's side outputs require.
+ * No counterpart in the reference. */
+int sol_set_side_outputs(sol_context *ctx, int32_t enable);
 
 /* ---- integrators ------------------------------------------------------ */
 /* Upload BodyData::y0 (6*total doubles, host). */
@@ -252,6 +260,9 @@ typedef struct sol_kernel_profile {
     double  epilogue_ms;
     int64_t pair_nn_launches;  /* pair-sum launches that also track the nearest neighbour */
     double  pair_nn_ms;
+    int64_t pair_sym_launches; /* symmetric pair sums of the self-interacting block */
+    double  pair_sym_ms;
+    double  pair_sym_interactions; /* interactions they formed (two per pair) */
 } sol_kernel_profile;
 /* enable = 1 clears the totals and starts bracketing every pair-sum and
  * epilogue launch with HIP events on the context's stream; enable = 0

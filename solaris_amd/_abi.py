@@ -50,7 +50,9 @@ class RunStats(C.Structure):
 class KernelProfile(C.Structure):
     _fields_ = [("pair_launches", C.c_int64), ("pair_ms", C.c_double),
                 ("pair_interactions", C.c_double), ("epilogue_launches", C.c_int64),
-                ("epilogue_ms", C.c_double), ("pair_nn_launches", C.c_int64), ("pair_nn_ms", C.c_double)]
+                ("epilogue_ms", C.c_double), ("pair_nn_launches", C.c_int64), ("pair_nn_ms", C.c_double),
+                ("pair_sym_launches", C.c_int64), ("pair_sym_ms", C.c_double),
+                ("pair_sym_interactions", C.c_double)]
 
 
 def nbodies(counts: dict) -> NBodies:

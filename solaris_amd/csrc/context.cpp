@@ -112,37 +112,22 @@ int n_pl_spl(const sol_nbodies &nb) { return nb.super_planetesimal + nb.planetes
 
 bool mixed_on(const sol_context *c) { return c->precision == SOL_PRECISION_MIXED && c->J32 != nullptr; }
 
-// Builds the pair-sum work plan for the context's sink classes.
-void build_plan(sol_context *c)
+struct SegSpec { int sb, se, jb, je; };
+
+// Stream-K work plan over sink ranges x source ranges; *pairs receives the
+// pair terms one launch evaluates (self pairs excluded).
+PairPlan make_plan(const sol_context *c, const SegSpec *segs, int ns, double *pairs)
 {
     PairPlan p{};
-    struct S { int sb, se, jb, je; } segs[2];
-    int ns = 0;
-    if (c->barycentric) {
-        segs[ns++] = {0, c->n_massive, 0, c->n_massive};
-        segs[ns++] = {c->n_massive, c->n, 0, c->n_massive};
-    } else {
-        int ea, eb;
-        source_ends(c, ea, eb);
-        segs[ns++] = {1, c->n_massive, 1, ea};
-        segs[ns++] = {c->n_massive, c->n, 1, eb};
-    }
     int64_t items = 0;
     int tiles = 0;
-    double inter = 0.0;
     p.nseg = 0;
     p.chunk = c->chunk;
+    double np = 0.0;
     for (int k = 0; k < ns; ++k) {
-        const S &s = segs[k];
+        const SegSpec &s = segs[k];
         const int nsink = s.se - s.sb, nsrc = s.je - s.jb;
-        if (nsink <= 0) continue;
-        // interactions: pair terms (minus self where the sink is a source) + Kepler (AC)
-        for (int pass = 0; pass < 1; ++pass) {
-            const int lo = std::max(s.sb, s.jb), hi = std::min(s.se, s.je);
-            const double self = hi > lo ? (double)(hi - lo) : 0.0;
-            inter += (double)nsink * (double)std::max(nsrc, 0) - self + (c->barycentric ? 0.0 : (double)nsink);
-        }
-        if (nsrc <= 0) continue;
+        if (nsink <= 0 || nsrc <= 0) continue;
         PairSeg &g = p.seg[p.nseg++];
         g.sink_begin = s.sb; g.sink_end = s.se;
         g.src_begin = s.jb; g.src_end = s.je;
@@ -152,21 +137,18 @@ void build_plan(sol_context *c)
         g.item_begin = items;
         tiles += g.ntiles;
         items += (int64_t)g.ntiles * g.nchunks;
+        const int lo = std::max(s.sb, s.jb), hi = std::min(s.se, s.je);
+        np += (double)nsink * (double)nsrc - (hi > lo ? hi - lo : 0);
     }
     p.ntiles = tiles;
     p.total_items = items;
-    double pairs = 0.0;
-    for (int k = 0; k < p.nseg; ++k) {
-        const PairSeg &g = p.seg[k];
-        const int lo = std::max(g.sink_begin, g.src_begin), hi = std::min(g.sink_end, g.src_end);
-        pairs += (double)(g.sink_end - g.sink_begin) * (double)(g.src_end - g.src_begin) - (hi > lo ? hi - lo : 0);
-    }
-    c->pair_inter_per_launch = pairs;
+    if (pairs) *pairs = np;
     const int64_t wmax = (int64_t)c->cu_count * 4 * c->waves_per_simd;
     p.nwaves = (int)std::max<int64_t>(1, std::min<int64_t>(wmax, items));
     // slots per wave: the largest number of tiles one wave's range touches
     auto tile_of = [&](int64_t x) {
-        const int s = (p.nseg > 1 && x >= p.seg[1].item_begin) ? 1 : 0;
+        int s = 0;
+        while (s + 1 < p.nseg && x >= p.seg[s + 1].item_begin) ++s;
         return p.seg[s].tile_begin + (int)((x - p.seg[s].item_begin) / p.seg[s].nchunks);
     };
     int spw = 1;
@@ -177,13 +159,81 @@ void build_plan(sol_context *c)
         }
     }
     p.slots_per_wave = spw;
-    c->plan = p;
-    c->interactions_per_eval = inter;
+    return p;
+}
+
+// Bodies of the mutual block below which the one-directional sum is kept.
+constexpr int kSymMinBodies = 2048;
+
+// Builds the pair-sum work plans for the context's sink classes: the full
+// one-directional plan (every evaluation that tracks nearest neighbours,
+// the mixed form, barycentric frames) and, when the self-interacting block
+// [1, M) is large, the symmetric plan of that block plus a one-directional
+// plan of the remaining (sink, source) ranges.
+void build_plan(sol_context *c)
+{
+    SegSpec segs[kMaxSeg];
+    int ns = 0;
+    int ea = 0, eb = 0;
+    if (c->barycentric) {
+        segs[ns++] = {0, c->n_massive, 0, c->n_massive};
+        segs[ns++] = {c->n_massive, c->n, 0, c->n_massive};
+    } else {
+        source_ends(c, ea, eb);
+        segs[ns++] = {1, c->n_massive, 1, ea};
+        segs[ns++] = {c->n_massive, c->n, 1, eb};
+    }
+    double pairs = 0.0;
+    c->plan = make_plan(c, segs, ns, &pairs);
+    c->pair_inter_per_launch = pairs;
+    // interactions per evaluation: pair terms plus the Kepler term of every
+    // astrocentric sink
+    double kepler = 0.0;
+    for (int k = 0; k < ns; ++k)
+        if (!c->barycentric && segs[k].se > segs[k].sb) kepler += (double)(segs[k].se - segs[k].sb);
+    c->interactions_per_eval = pairs + kepler;
+
+    c->sym = SymPlan{};
+    c->plan_rest = PairPlan{};
+    c->rest_inter_per_launch = 0.0;
+    c->sym_inter_per_launch = 0.0;
+    if (c->barycentric || c->sym_disabled) return;
+    const int nm = c->n_massive;
+    // largest M such that every body of [1, M) is a source of every other
+    const int M = std::min(ea, eb) > nm ? std::min(std::min(ea, eb), c->n) : std::min(nm, ea);
+    if (M - 1 < kSymMinBodies) return;
+    const int64_t waves = (int64_t)c->cu_count * 4 * 4;
+    sym_plan_make(c->sym, M, waves);
+    c->sym_inter_per_launch = (double)(M - 1) * (double)(M - 2);
+    SegSpec rest[kMaxSeg];
+    int nr = 0;
+    rest[nr++] = {1, std::min(M, nm), M, ea};                 // mutual sinks of the first class, sources past M
+    if (M > nm) rest[nr++] = {nm, M, M, eb};                  // mutual sinks of the second class, sources past M
+    if (M < nm) rest[nr++] = {M, nm, 1, ea};                  // first-class sinks outside the block
+    rest[nr++] = {std::max(M, nm), c->n, 1, eb};              // second-class sinks outside the block
+    c->plan_rest = make_plan(c, rest, nr, &c->rest_inter_per_launch);
 }
 
 int ensure_partials(sol_context *c)
 {
-    const size_t need = (size_t)c->plan.nwaves * c->plan.slots_per_wave;
+    if (c->sym.M > 0) {
+        const size_t na = (size_t)c->sym.nitems * 3 * kSymGroup;
+        const size_t nb = (size_t)c->sym.nwaves * c->sym.slots_per_wave * 3 * kSymGroup;
+        if (na > c->apart_cap) {
+            if (c->apart) (void)hipFree(c->apart);
+            c->apart = nullptr; c->apart_cap = 0;
+            HIPCHECK(c, hipMalloc(&c->apart, na * sizeof(double)));
+            c->apart_cap = na;
+        }
+        if (nb > c->bpart_cap) {
+            if (c->bpart) (void)hipFree(c->bpart);
+            c->bpart = nullptr; c->bpart_cap = 0;
+            HIPCHECK(c, hipMalloc(&c->bpart, nb * sizeof(double)));
+            c->bpart_cap = nb;
+        }
+    }
+    const size_t need = std::max((size_t)c->plan.nwaves * c->plan.slots_per_wave,
+                                 (size_t)c->plan_rest.nwaves * c->plan_rest.slots_per_wave);
     if (need <= c->part_slots) return SOL_OK;
     if (c->part) { (void)hipFree(c->part); (void)hipFree(c->nnr2); (void)hipFree(c->nnidx); }
     c->part = nullptr; c->nnr2 = nullptr; c->nnidx = nullptr;
@@ -205,11 +255,14 @@ hipEvent_t prof_event(sol_context *c)
     return c->ev_pool[c->ev_used++];
 }
 
+// kinds: 0 one-directional pair sum, 1 epilogue, 2 pair sum with the
+// nearest-neighbour search, 3 symmetric pair sum
 struct ProfScope {
     sol_context *c;
     hipEvent_t a = nullptr;
     int kind;
-    ProfScope(sol_context *c_, int kind_) : c(c_), kind(kind_)
+    double inter;
+    ProfScope(sol_context *c_, int kind_, double inter_ = 0.0) : c(c_), kind(kind_), inter(inter_)
     {
         if (c->profiling && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
     }
@@ -219,7 +272,7 @@ struct ProfScope {
         hipEvent_t b = prof_event(c);
         if (!b) return;
         (void)hipEventRecord(b, c->stream);
-        c->timed.push_back({a, b, kind, kind != 1 ? c->pair_inter_per_launch : 0.0});
+        c->timed.push_back({a, b, kind, inter});
     }
 };
 
@@ -233,7 +286,9 @@ void prof_collect(sol_context *c)
         (void)hipEventElapsedTime(&ms, t.a, t.b);
         if (t.kind == 0) { c->prof.pair_launches++; c->prof.pair_ms += ms; c->prof.pair_interactions += t.inter; }
         else if (t.kind == 2) { c->prof.pair_nn_launches++; c->prof.pair_nn_ms += ms; }
-        else { c->prof.epilogue_launches++; c->prof.epilogue_ms += ms; }
+        else if (t.kind == 3) {
+            c->prof.pair_sym_launches++; c->prof.pair_sym_ms += ms; c->prof.pair_sym_interactions += t.inter;
+        } else { c->prof.epilogue_launches++; c->prof.epilogue_ms += ms; }
     }
     c->timed.clear();
     c->ev_used = 0;
@@ -288,6 +343,38 @@ int upload_scalars(sol_context *c, double h, double t)
     return SOL_OK;
 }
 
+// The pair sums of one evaluation from the current J: the symmetric sum of
+// the mutual block plus the rest when possible (FP64, no nearest-neighbour
+// search), else the one-directional sum.  Sets the epilogue's plan fields.
+void pair_sums(sol_context *c, bool nn, EpiArgs &a)
+{
+    const double eps2 = c->softening * c->softening;
+    if (c->sym.M > 0 && !nn && !mixed_on(c)) {
+        {
+            ProfScope ps(c, 3, c->sym_inter_per_launch);
+            launch_pairs_sym(c->stream, c->sym, c->J, c->apart, c->bpart, eps2);
+        }
+        if (c->plan_rest.total_items > 0) {
+            ProfScope ps(c, 0, c->rest_inter_per_launch);
+            launch_pairs(c->stream, c->plan_rest, c->J, c->part, c->nnr2, c->nnidx, false, false, false, eps2,
+                         c->J32);
+        }
+        a.plan = c->plan_rest;
+        a.sym = c->sym;
+        a.use_sym = 1;
+        a.apart = c->apart;
+        a.bpart = c->bpart;
+        return;
+    }
+    {
+        ProfScope ps(c, nn ? 2 : 0, c->pair_inter_per_launch);
+        launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, nn, c->barycentric != 0, mixed_on(c),
+                     eps2, c->J32);
+    }
+    a.plan = c->plan;
+    a.use_sym = 0;
+}
+
 // One full force evaluation of the state `st` (6 x NP) at time t, the
 // reference's Acceleration::Compute, with the derivative in `out`.
 // kind: EPI_COMPUTE or EPI_STAGE0 (the Driver's evaluation at y0; also
@@ -309,12 +396,8 @@ int evaluate_at(sol_context *c, const double *st, double t, double h, double *ou
             launch_migration(c->stream, c->nb.central_body, c->nb.giant_planet, c->n_massive, c->np, st, c->mass,
                              c->stop, c->migt, c->migc, c->gasdev, c->barycentric, t, do1, do2);
     }
-    {
-        ProfScope ps(c, last ? 2 : 0);
-        launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
-                     mixed_on(c), c->softening * c->softening, c->J32);
-    }
     EpiArgs a = base_args(c);
+    pair_sums(c, last, a);
     a.kind = kind;
     a.integ = integ;
     a.stage = 0;
@@ -366,12 +449,8 @@ int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double
     launch_combine_first(c->stream, a);
     for (int s = 1; s <= S; ++s) {
         const bool last = (s == S);
-        {
-            ProfScope ps(c, last ? 2 : 0);
-            launch_pairs(c->stream, c->plan, c->J, c->part, c->nnr2, c->nnidx, last, c->barycentric != 0,
-                         mixed_on(c), c->softening * c->softening, c->J32);
-        }
         EpiArgs e = a;
+        pair_sums(c, last, e);
         e.stage = s;
         e.kind = last ? EPI_FINAL : EPI_MID;
         e.last = last ? 1 : 0;
@@ -601,6 +680,7 @@ int sol_create(sol_context **out, const sol_nbodies *nb, int32_t barycentric, in
         c->lds_limit = prop.sharedMemPerBlock > 2048 ? prop.sharedMemPerBlock - 2048 : 0;
     }
     if (const char *e = std::getenv("SOLARIS_AMD_NO_RESIDENT")) c->resident_disabled = e[0] == '1';
+    if (const char *e = std::getenv("SOLARIS_AMD_NO_SYM")) c->sym_disabled = e[0] == '1';
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(SOL_ERR_DEVICE);
@@ -652,6 +732,8 @@ int sol_destroy(sol_context *c)
     if (c->part) (void)hipFree(c->part);
     if (c->nnr2) (void)hipFree(c->nnr2);
     if (c->nnidx) (void)hipFree(c->nnidx);
+    if (c->apart) (void)hipFree(c->apart);
+    if (c->bpart) (void)hipFree(c->bpart);
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->res_h) (void)hipHostFree(c->res_h);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -738,6 +820,13 @@ int sol_set_precision(sol_context *c, int32_t precision, double softening)
     return SOL_OK;
 }
 
+int sol_set_side_outputs(sol_context *c, int32_t enable)
+{
+    if (check_ctx(c)) return SOL_ERR_ARGUMENT;
+    c->side_outputs = enable != 0;
+    return SOL_OK;
+}
+
 int sol_set_source_ranges(sol_context *c, int32_t a, int32_t b)
 {
     if (check_ctx(c)) return SOL_ERR_ARGUMENT;
@@ -756,7 +845,7 @@ int sol_compute(sol_context *c, double t, const double *yh, double *dydt)
     const size_t bytes = 6 * (size_t)c->n * sizeof(double);
     HIPCHECK(c, hipMemcpyAsync(c->aos, yh, bytes, hipMemcpyHostToDevice, c->stream));
     launch_aos_to_soa(c->stream, c->n, c->np, c->aos, c->cin);
-    int r = evaluate_at(c, c->cin, t, 0.0, c->cout, EPI_COMPUTE, INTEG_RKF78, true, false);
+    int r = evaluate_at(c, c->cin, t, 0.0, c->cout, EPI_COMPUTE, INTEG_RKF78, c->side_outputs, false);
     if (r) return r;
     launch_soa_to_aos(c->stream, c->n, c->np, c->cout, c->aos);
     HIPCHECK(c, hipMemcpyAsync(dydt, c->aos, bytes, hipMemcpyDeviceToHost, c->stream));

@@ -64,6 +64,15 @@ struct sol_context {
     size_t ev_used = 0;
     sol_kernel_profile prof{};
     double pair_inter_per_launch = 0.0;
+    // symmetric pair sum of the self-interacting block (pairs.hip) and the
+    // one-directional plan of the remaining ranges
+    sol::SymPlan sym{};
+    sol::PairPlan plan_rest{};
+    double sym_inter_per_launch = 0.0, rest_inter_per_launch = 0.0;
+    double *apart = nullptr, *bpart = nullptr;
+    size_t apart_cap = 0, bpart_cap = 0;
+    bool sym_disabled = false;
+    bool side_outputs = true;     // sol_compute tracks nearest neighbours (sol_set_side_outputs)
     // recorded forward pass of the RK4 adjoint (adjoint.cpp)
     sol::AdjointState *adj = nullptr;
 

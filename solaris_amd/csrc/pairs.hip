@@ -295,7 +295,149 @@ void launch_mixed(hipStream_t st, const PairPlan &plan, const float4 *J32, const
                        plan, J32, J, part, nnr2, nnidx, (float)eps2, eps2, kNearFrac * kNearFrac, 1.5, 1.875);
 }
 
+// ---- symmetric pair sum over the mutual block ------------------------------
+// One wave per range of group-pair items (sym_plan.h).  The wave holds the
+// 256 bodies of the row group, four per lane, and streams the column group
+// as four packets of 64 bodies, one per lane, that rotate one lane per step
+// (DPP wave_rol:1): after 64 steps every row body has met every packet body
+// and the packet, which carries its own force sums, is back on its lane.
+// Each step evaluates four pairs per lane and forms both accelerations from
+// each, so a pair of the block costs one reciprocal square root and 20 FP64
+// operations for two interactions, plus the packet's 14 rotated dwords per
+// step (3.5 per pair).  In a diagonal item (row == column) the row tile t
+// meets packet j only for t < j, and for t == j at steps 1..32 (step 32 on
+// lanes 0..31): each unordered pair once, no self pair.
+// Bodies past the end of the block are far sentinels of zero mass at
+// +1e30 AU (row side) and -1e30 AU (column side), so that no pair has zero
+// distance; their sums are never read.
+
+__device__ __forceinline__ double rot_lane(double v)
+{
+    const int2 p = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_mov_dpp(p.x, 0x134, 0xF, 0xF, false);   // wave_rol:1, lane l <- lane l+1
+    r.y = __builtin_amdgcn_mov_dpp(p.y, 0x134, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, r);
+}
+
+template <bool SOFT>
+__device__ __forceinline__ void sym_pair(double x, double y, double z, double gma, double4 q, double eps2,
+                                         double c15, double c1875, double &fx, double &fy, double &fz,
+                                         double &gx, double &gy, double &gz)
+{
+    const double dx = q.x - x, dy = q.y - y, dz = q.z - z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double rs = SOFT ? r2 + eps2 : r2;
+    const double y0 = __builtin_amdgcn_rsq(rs);
+    const double tt = y0 * y0;
+    const double e = __builtin_fma(-rs, tt, 1.0);
+    const double y3 = tt * y0;
+    const double w = __builtin_fma(y3, e * __builtin_fma(c1875, e, c15), y3);   // |d|^-3
+    const double fa = q.w * w, fb = gma * w;
+    fx = __builtin_fma(fa, dx, fx);
+    fy = __builtin_fma(fa, dy, fy);
+    fz = __builtin_fma(fa, dz, fz);
+    gx = __builtin_fma(-fb, dx, gx);
+    gy = __builtin_fma(-fb, dy, gy);
+    gz = __builtin_fma(-fb, dz, gz);
+}
+
+template <bool SOFT>
+__global__ __launch_bounds__(256) void k_pairs_sym_fp64(SymPlan sp, const double4 *__restrict__ J,
+                                                        double *__restrict__ apart, double *__restrict__ bpart,
+                                                        double eps2, double c15, double c1875)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (w >= sp.nwaves) return;
+    const int64_t lo = sym_wave_lo(sp, w), hi = sym_wave_lo(sp, w + 1);
+    if (lo >= hi) return;
+    int a0, h0;
+    sym_decode(lo, a0, h0);
+    int hcur = -1;
+    double bs[4][3];                         // column-side sums of the current column group
+
+    auto flush_col = [&]() {
+        double *q = bpart + ((int64_t)w * sp.slots_per_wave + (hcur - h0)) * (3 * kSymGroup);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c * kSymGroup + t * kWave + lane] = bs[t][c];
+    };
+
+    for (int64_t it = lo; it < hi; ++it) {
+        int ag, hg;
+        sym_decode(it, ag, hg);
+        if (hg != hcur) {
+            if (hcur >= 0) flush_col();
+            hcur = hg;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bs[t][0] = bs[t][1] = bs[t][2] = 0.0;
+        }
+        double px[4], py[4], pz[4], pm[4], fx[4], fy[4], fz[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int ia = 1 + ag * kSymGroup + t * kWave + lane;
+            const double4 v = ia < sp.M ? J[ia] : make_double4(1.0e30, 0.0, 0.0, 0.0);
+            px[t] = v.x; py[t] = v.y; pz[t] = v.z; pm[t] = v.w;
+            fx[t] = fy[t] = fz[t] = 0.0;
+        }
+        const bool diag = (ag == hg);
+        for (int j = 0; j < 4; ++j) {
+            const int ib = 1 + hg * kSymGroup + j * kWave + lane;
+            double4 q = ib < sp.M ? J[ib] : make_double4(-1.0e30, 0.0, 0.0, 0.0);
+            double gx = 0.0, gy = 0.0, gz = 0.0;
+            if (!diag) {
+                for (int k = 0; k < kWave; ++k) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        sym_pair<SOFT>(px[t], py[t], pz[t], pm[t], q, eps2, c15, c1875, fx[t], fy[t], fz[t],
+                                       gx, gy, gz);
+                    q.x = rot_lane(q.x); q.y = rot_lane(q.y); q.z = rot_lane(q.z); q.w = rot_lane(q.w);
+                    gx = rot_lane(gx); gy = rot_lane(gy); gz = rot_lane(gz);
+                }
+            } else {
+                for (int k = 0; k < kWave; ++k) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const bool on = t < j || (t == j && k >= 1 && (k < 32 || (k == 32 && lane < 32)));
+                        if (on)
+                            sym_pair<SOFT>(px[t], py[t], pz[t], pm[t], q, eps2, c15, c1875, fx[t], fy[t],
+                                           fz[t], gx, gy, gz);
+                    }
+                    q.x = rot_lane(q.x); q.y = rot_lane(q.y); q.z = rot_lane(q.z); q.w = rot_lane(q.w);
+                    gx = rot_lane(gx); gy = rot_lane(gy); gz = rot_lane(gz);
+                }
+            }
+            bs[j][0] += gx;
+            bs[j][1] += gy;
+            bs[j][2] += gz;
+        }
+        double *r = apart + it * (3 * kSymGroup);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            r[t * kWave + lane] = fx[t];
+            r[kSymGroup + t * kWave + lane] = fy[t];
+            r[2 * kSymGroup + t * kWave + lane] = fz[t];
+        }
+    }
+    if (hcur >= 0) flush_col();
+}
+
 }  // namespace
+
+void launch_pairs_sym(hipStream_t st, const SymPlan &sp, const double4 *J, double *apart, double *bpart,
+                      double eps2)
+{
+    if (sp.M <= 2) return;
+    const int blocks = (sp.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
+    if (eps2 != 0.0)
+        hipLaunchKernelGGL(k_pairs_sym_fp64<true>, dim3(blocks), dim3(kBlock), 0, st, sp, J, apart, bpart, eps2,
+                           1.5, 1.875);
+    else
+        hipLaunchKernelGGL(k_pairs_sym_fp64<false>, dim3(blocks), dim3(kBlock), 0, st, sp, J, apart, bpart, eps2,
+                           1.5, 1.875);
+}
 
 void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
                   double *nnr2, int32_t *nnidx, bool nn, bool bc, bool mixed, double eps2,

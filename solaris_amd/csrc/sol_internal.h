@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/solaris_amd.h"
+#include "sym_plan.h"
 
 namespace sol {
 
@@ -20,7 +21,7 @@ constexpr double kGauss  = 1.720209895e-2;           // Solaris/Constants.h:28
 constexpr double kGauss2 = 2.959122082855911025e-4;  // Solaris/Constants.h:29
 constexpr int kBlock = 256;   // threads per block of the elementwise kernels
 constexpr int kWave = 64;     // CDNA wavefront
-constexpr int kMaxSeg = 2;    // sink classes per force evaluation
+constexpr int kMaxSeg = 4;    // sink ranges per pair-sum launch
 
 // One sink class: sinks [sink_begin, sink_end) see sources
 // [src_begin, src_end) (minus themselves).  Sinks are cut into tiles of 64
@@ -94,6 +95,9 @@ struct EpiArgs {
     double *yscale_out, *y_out, *err_out, *errblk;
     const double *scal;       // device scalars: [0] = h, [1] = t
     const double *coef;       // integrator coefficient tables (context.cpp)
+    SymPlan sym;              // symmetric pair sum of the mutual block (used when use_sym)
+    int32_t use_sym, pad3;
+    const double *apart, *bpart;   // its row-side and column-side records
 };
 
 // Arguments of one force vector-Jacobian product (adjoint.hip).  Y is the
@@ -124,6 +128,10 @@ void fill_coefficients(std::vector<double> &c);
 void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
                   double *nnr2, int32_t *nnidx, bool nn, bool barycentric, bool mixed,
                   double eps2, const float4 *J32);
+// Symmetric pair sum over the mutual block (sym_plan.h): row-side records
+// in apart, column-side records in bpart.
+void launch_pairs_sym(hipStream_t st, const SymPlan &sp, const double4 *J, double *apart, double *bpart,
+                      double eps2);
 void launch_prep(hipStream_t st, int n, int np, const double *state, const double *gm,
                  double4 *J, float4 *J32, double *ind_out, int barycentric, int src_end_a,
                  int src_end_b);

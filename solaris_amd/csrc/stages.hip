@@ -307,6 +307,32 @@ __device__ bool gather_pairs(const EpiArgs &a, int i, double &px, double &py, do
     const PairPlan &p = a.plan;
     px = py = pz = 0.0;
     best = DBL_MAX; bidx = -1;
+    bool found = false;
+    if (a.use_sym && i >= 1 && i < a.sym.M) {
+        // symmetric sum of the mutual block: the row records of items (g, h),
+        // h >= g, then the column records of the waves that cover column g
+        const SymPlan &sp = a.sym;
+        const int g = (i - 1) / kSymGroup, r = (i - 1) - g * kSymGroup;
+        for (int h = g; h < sp.ngroups; ++h) {
+            const double *q = a.apart + (sym_col_start(h) + g) * (3 * kSymGroup);
+            px += q[r];
+            py += q[kSymGroup + r];
+            pz += q[2 * kSymGroup + r];
+        }
+        const int64_t A = sym_col_start(g), B = A + g + 1;
+        const int64_t w0 = sym_wave_of(sp, A), w1 = sym_wave_of(sp, B - 1);
+        for (int64_t w = w0; w <= w1; ++w) {
+            const int64_t lo = sym_wave_lo(sp, w), hi = sym_wave_lo(sp, w + 1);
+            if (lo >= hi || hi <= A || lo >= B) continue;
+            int a0, h0;
+            sym_decode(lo, a0, h0);
+            const double *q = a.bpart + (w * sp.slots_per_wave + (g - h0)) * (3 * kSymGroup);
+            px += q[r];
+            py += q[kSymGroup + r];
+            pz += q[2 * kSymGroup + r];
+        }
+        found = true;
+    }
     for (int s = 0; s < p.nseg; ++s) {
         const PairSeg &sg = p.seg[s];
         if (i < sg.sink_begin || i >= sg.sink_end) continue;
@@ -337,7 +363,7 @@ __device__ bool gather_pairs(const EpiArgs &a, int i, double &px, double &py, do
         }
         return true;
     }
-    return false;
+    return found;
 }
 
 // Stage derivative component c of stage l for body i.

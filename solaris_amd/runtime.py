@@ -47,6 +47,7 @@ _SIGS = {
     "sol_compute": (C.c_int, [V, C.c_double, D, D]),
     "sol_get_side_outputs": (C.c_int, [V, D, I, D, I]),
     "sol_set_mig_type": (C.c_int, [V, I]),
+    "sol_set_side_outputs": (C.c_int, [V, C.c_int32]),
     "sol_set_state": (C.c_int, [V, D]),
     "sol_get_array": (C.c_int, [V, C.c_int32, D]),
     "sol_set_accel": (C.c_int, [V, D]),
@@ -164,6 +165,9 @@ class Context:
 
     def set_source_ranges(self, a, b):
         self._check(self.lib.sol_set_source_ranges(self.h, int(a), int(b)))
+
+    def set_side_outputs(self, enable: bool):
+        self._check(self.lib.sol_set_side_outputs(self.h, int(bool(enable))))
 
     def set_mig_type(self, mig):
         m = np.ascontiguousarray(mig, dtype=np.int32)

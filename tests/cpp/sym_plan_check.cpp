@@ -1,0 +1,93 @@
+// Host-side check of the symmetric pair-sum plan (solaris_amd/csrc/sym_plan.h):
+// item numbering, wave ranges, column-record slots and the epilogue's lookup.
+// Exit status 0 when every check holds; prints the first failure otherwise.
+#include <cstdio>
+#include <map>
+#include <set>
+#include <utility>
+#include <vector>
+
+#include "../../solaris_amd/csrc/sym_plan.h"
+
+using namespace sol;
+
+static int fail(const char *what, long long a, long long b, long long c)
+{
+    std::printf("FAIL %s (%lld, %lld, %lld)\n", what, a, b, c);
+    return 1;
+}
+
+static int check(int M, long long waves)
+{
+    SymPlan p;
+    sym_plan_make(p, M, waves);
+    const int G = p.ngroups;
+    if (p.nitems != (long long)G * (G + 1) / 2) return fail("nitems", M, waves, p.nitems);
+    // 1. decode is the inverse of the numbering; every unordered group pair once
+    std::set<std::pair<int, int>> seen;
+    for (long long it = 0; it < p.nitems; ++it) {
+        int a, h;
+        sym_decode(it, a, h);
+        if (a < 0 || a > h || h >= G) return fail("decode range", M, it, h);
+        if (sym_col_start(h) + a != it) return fail("decode inverse", M, it, a);
+        if (!seen.insert({a, h}).second) return fail("pair twice", M, a, h);
+    }
+    if ((long long)seen.size() != p.nitems) return fail("pairs missing", M, waves, seen.size());
+    // 2. wave ranges and wave_of; 3. column records written by each wave
+    std::map<std::pair<long long, int>, int> written;   // (wave, slot) -> column
+    long long covered = 0;
+    for (long long w = 0; w < p.nwaves; ++w) {
+        const long long lo = sym_wave_lo(p, w), hi = sym_wave_lo(p, w + 1);
+        if (hi < lo) return fail("range order", M, w, lo);
+        covered += hi - lo;
+        if (lo == hi) continue;
+        int a0, h0;
+        sym_decode(lo, a0, h0);
+        for (long long it = lo; it < hi; ++it) {
+            if (sym_wave_of(p, it) != w) return fail("wave_of", M, it, w);
+            int a, h;
+            sym_decode(it, a, h);
+            const int slot = h - h0;
+            if (slot < 0 || slot >= p.slots_per_wave) return fail("slot range", M, w, slot);
+            auto key = std::make_pair(w, slot);
+            auto f = written.find(key);
+            if (f != written.end() && f->second != h) return fail("slot reuse", M, w, slot);
+            written[key] = h;
+        }
+    }
+    if (covered != p.nitems) return fail("coverage", M, waves, covered);
+    // 4. the epilogue's lookup of column g finds exactly the records written for g
+    for (int g = 0; g < G; ++g) {
+        std::set<std::pair<long long, int>> want, got;
+        for (auto &kv : written)
+            if (kv.second == g) want.insert(kv.first);
+        const long long A = sym_col_start(g), B = A + g + 1;
+        for (long long w = sym_wave_of(p, A); w <= sym_wave_of(p, B - 1); ++w) {
+            const long long lo = sym_wave_lo(p, w), hi = sym_wave_lo(p, w + 1);
+            if (lo >= hi || hi <= A || lo >= B) continue;
+            int a0, h0;
+            sym_decode(lo, a0, h0);
+            got.insert({w, g - h0});
+        }
+        if (want != got) return fail("column lookup", M, g, (long long)got.size());
+        // row side: items (g, h), h >= g, all distinct
+        for (int h = g; h < G; ++h) {
+            int a, hh;
+            sym_decode(sym_col_start(h) + g, a, hh);
+            if (a != g || hh != h) return fail("row item", M, g, h);
+        }
+    }
+    return 0;
+}
+
+int main()
+{
+    const int Ms[] = {3, 4, 257, 258, 513, 1000, 4097, 10000, 65537, 262145};
+    const long long Ws[] = {1, 3, 7, 64, 1000, 4096, 1 << 20};
+    int bad = 0;
+    for (int M : Ms)
+        for (long long W : Ws)
+            if (check(M, W)) { bad = 1; break; }
+    std::printf(bad ? "sym plan: FAILED\n" : "sym plan: ok\n");
+    return bad;
+}

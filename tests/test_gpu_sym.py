@@ -1,0 +1,120 @@
+"""The symmetric pair sum of the self-interacting block (k_pairs_sym_fp64).
+
+Evaluations without the nearest-neighbour search (sol_compute with side
+outputs off, and every stage of an attempt but the last) evaluate each pair
+of the mutual block [1, M) once for both bodies, and the remaining
+(sink, source) ranges one-directionally.  Checks, as for the plain path:
+accelerations within 1e-12 relative of the oracle, on a self-gravitating
+disk (with and without Plummer softening), on protoplanets plus
+planetesimals with gas drag and Type I migration (symmetric block plus a
+one-directional rest), and on mutually attracting planetesimals; the
+symmetric and one-directional paths within 1e-14 of each other; replayed
+RKF78 trajectories within 1e-9 AU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.cpu_ref import Oracle
+from solaris_amd import _abi, runtime, scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    num = np.linalg.norm(a - b, axis=1)
+    den = np.linalg.norm(b, axis=1)
+    return np.where(den > 0, num / np.where(den > 0, den, 1.0), num)
+
+
+def compute_no_side_outputs(scn, t=0.0, ranges=None, evaluate=None, no_sym=False):
+    if no_sym:
+        os.environ["SOLARIS_AMD_NO_SYM"] = "1"
+    try:
+        ctx = runtime.Context(scn)
+    finally:
+        os.environ.pop("SOLARIS_AMD_NO_SYM", None)
+    if ranges:
+        ctx.set_source_ranges(*ranges)
+    if evaluate:
+        ctx.set_evaluate(*evaluate)
+    ctx.set_side_outputs(False)
+    a = ctx.compute(t, scn.y0)
+    ctx.close()
+    return a
+
+
+def oracle_compute(scn, t=0.0, ranges=None, evaluate=None):
+    o = Oracle(scn, lib="omp")
+    if ranges:
+        o.set_source_ranges(*ranges)
+    if evaluate:
+        o.set_evaluate(*evaluate)
+    return o.compute(t, scn.y0)
+
+
+@pytest.mark.parametrize("softening", [0.0, 1.0e-5])
+def test_sym_compute_self_gravitating_disk(softening):
+    scn = S.make_c3(n=4096, softening=softening)
+    print(f"[sym c3 n={scn.n} eps={softening}]", flush=True)
+    a = compute_no_side_outputs(scn)
+    b = oracle_compute(scn)
+    assert np.array_equal(a[:, :3], b[:, :3])
+    err = rel_err(a[:, 3:], b[:, 3:])
+    print(f"  max rel err {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-12
+
+
+def test_sym_compute_protoplanets_and_planetesimals_with_gas():
+    scn = S.make_c2(n_pl=1500, n_proto=2500)
+    print(f"[sym c2 protoplanets {scn.n_massive()} + planetesimals]", flush=True)
+    a = compute_no_side_outputs(scn, t=10.0, evaluate=(1, 1, 1))
+    b = oracle_compute(scn, t=10.0, evaluate=(1, 1, 1))
+    err = rel_err(a[:, 3:], b[:, 3:])
+    print(f"  max rel err {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-12
+
+
+def test_sym_compute_mutually_attracting_planetesimals():
+    scn = S.make_c2(n_pl=3000)
+    ranges = (scn.n, scn.n)
+    print(f"[sym c2-self n={scn.n}]", flush=True)
+    a = compute_no_side_outputs(scn, t=10.0, ranges=ranges)
+    b = oracle_compute(scn, t=10.0, ranges=ranges)
+    err = rel_err(a[:, 3:], b[:, 3:])
+    print(f"  max rel err {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-12
+
+
+def test_sym_and_one_directional_paths_agree():
+    scn = S.make_c3(n=6000, softening=0.0)
+    a = compute_no_side_outputs(scn)
+    b = compute_no_side_outputs(scn, no_sym=True)
+    err = rel_err(a[:, 3:], b[:, 3:])
+    print(f"  symmetric vs one-directional: max rel diff {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-14
+
+
+def test_sym_rkf78_replay():
+    scn = S.make_c3(n=2600, softening=0.0)
+    ctx = runtime.Context(scn)
+    orc = Oracle(scn, lib="omp")
+    ctx.set_state(scn.y0)
+    orc.set_state(scn.y0)
+    to = _abi.timeline(0.0, scn.h_next, 1e12)
+    ctx.profile(True)
+    worst = 0.0
+    for k in range(10):
+        assert orc.driver(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, to)[0] == 0
+        tg = _abi.timeline(to.time - to.hDid, to.hDid, 1e12)
+        r, _ = ctx.driver(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, tg)
+        assert r == 0, ctx.last_error()
+        assert tg.hDid == to.hDid
+        d = np.abs(ctx.get_array(runtime.Y0)[:, :3] - orc.get_array(runtime.Y0)[:, :3]).max()
+        worst = max(worst, d)
+    prof = ctx.profile(False)
+    print(f"  worst position difference over 10 steps: {worst:.3e} AU; symmetric launches {prof.pair_sym_launches}",
+          flush=True)
+    assert prof.pair_sym_launches > 0
+    assert worst <= 1e-9
