@@ -286,7 +286,10 @@ def main():
                              "steps_per_s": m["steps"] / m["wall_s"],
                              "softening_au": 1e-5,
                              "speedup_vs_fp64": (m["interactions"] / m["wall_s"]) / value,
-                             "pair_ms_per_launch": m["pair_ms"] / max(m["pair_launches"], 1)}
+                             "pair_kernel": "k_pairs_sym_mixed<true>" if m["pair_sym_launches"] else
+                                            "k_pairs_mixed<false, false, true>",
+                             "pair_ms_per_launch": (m["pair_sym_ms"] / m["pair_sym_launches"]) if m["pair_sym_launches"]
+                                                   else m["pair_ms"] / max(m["pair_launches"], 1)}
         except runtime.SolarisError as e:
             line["mixed"] = {"value": None, "note": str(e)}
     if not args.no_extras:

@@ -343,20 +343,24 @@ int upload_scalars(sol_context *c, double h, double t)
     return SOL_OK;
 }
 
-// The pair sums of one evaluation from the current J: the symmetric sum of
-// the mutual block plus the rest when possible (FP64, no nearest-neighbour
-// search), else the one-directional sum.  Sets the epilogue's plan fields.
+// The pair sums of one evaluation from the current J (and J32): the
+// symmetric sum of the mutual block plus the rest when possible (no
+// nearest-neighbour search), else the one-directional sum.  Sets the
+// epilogue's plan fields.
 void pair_sums(sol_context *c, bool nn, EpiArgs &a)
 {
     const double eps2 = c->softening * c->softening;
-    if (c->sym.M > 0 && !nn && !mixed_on(c)) {
+    if (c->sym.M > 0 && !nn) {
         {
             ProfScope ps(c, 3, c->sym_inter_per_launch);
-            launch_pairs_sym(c->stream, c->sym, c->J, c->apart, c->bpart, eps2);
+            if (mixed_on(c))
+                launch_pairs_sym_mixed(c->stream, c->sym, c->J32, c->J, c->apart, c->bpart, eps2);
+            else
+                launch_pairs_sym(c->stream, c->sym, c->J, c->apart, c->bpart, eps2);
         }
         if (c->plan_rest.total_items > 0) {
             ProfScope ps(c, 0, c->rest_inter_per_launch);
-            launch_pairs(c->stream, c->plan_rest, c->J, c->part, c->nnr2, c->nnidx, false, false, false, eps2,
+            launch_pairs(c->stream, c->plan_rest, c->J, c->part, c->nnr2, c->nnidx, false, false, mixed_on(c), eps2,
                          c->J32);
         }
         a.plan = c->plan_rest;

@@ -13,6 +13,8 @@
 // Kepler terms are added per sink in the epilogue (stages.hip).
 #include <float.h>
 
+#include <type_traits>
+
 #include "sol_internal.h"
 
 namespace sol {
@@ -424,7 +426,177 @@ __global__ __launch_bounds__(256) void k_pairs_sym_fp64(SymPlan sp, const double
     if (hcur >= 0) flush_col();
 }
 
+__device__ __forceinline__ float rot_lane_f(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x134, 0xF, 0xF, false));
+}
+
+// Mixed-precision form of k_pairs_sym_fp64: FP32 pair terms from J32 (and
+// FP32 sums over the 256 bodies an item pairs with each body, folded into
+// FP64), with the close pairs of k_pairs_mixed evaluated in FP64.  A pair
+// is close when |d|^2 < near_frac2 (|r_a|^2 + |r_b|^2) (same decision for
+// both bodies); the FP32 pass leaves it out and flags its lane, and after
+// the 64 steps of a packet a wave with a flagged lane replays the steps,
+// adding the FP64 terms of its close pairs (positions from J) to FP64 sums
+// that travel with the packet.  The packet rotates 8 dwords per step:
+// x, y, z, k^2 m, its close-pair threshold and its three FP32 sums.
+template <bool SOFT>
+__global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float4 *__restrict__ J32,
+                                                         const double4 *__restrict__ J, double *__restrict__ apart,
+                                                         double *__restrict__ bpart, float eps2, double eps2d,
+                                                         float near_frac2, double c15, double c1875)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (w >= sp.nwaves) return;
+    const int64_t lo = sym_wave_lo(sp, w), hi = sym_wave_lo(sp, w + 1);
+    if (lo >= hi) return;
+    int a0, h0;
+    sym_decode(lo, a0, h0);
+    int hcur = -1;
+    double bs[4][3];
+
+    auto flush_col = [&]() {
+        double *q = bpart + ((int64_t)w * sp.slots_per_wave + (hcur - h0)) * (3 * kSymGroup);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c * kSymGroup + t * kWave + lane] = bs[t][c];
+    };
+    auto active = [&](bool diag, int t, int j, int k) {
+        return !diag || t < j || (t == j && k >= 1 && (k < 32 || (k == 32 && lane < 32)));
+    };
+
+    for (int64_t it = lo; it < hi; ++it) {
+        int ag, hg;
+        sym_decode(it, ag, hg);
+        if (hg != hcur) {
+            if (hcur >= 0) flush_col();
+            hcur = hg;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) bs[t][0] = bs[t][1] = bs[t][2] = 0.0;
+        }
+        float px[4], py[4], pz[4], pm[4], th[4];
+        double ax[4], ay[4], az[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int ia = 1 + ag * kSymGroup + t * kWave + lane;
+            const float4 v = ia < sp.M ? J32[ia] : make_float4(1.0e18f, 0.0f, 0.0f, 0.0f);
+            px[t] = v.x; py[t] = v.y; pz[t] = v.z; pm[t] = v.w;
+            th[t] = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
+            ax[t] = ay[t] = az[t] = 0.0;
+        }
+        const bool diag = (ag == hg);
+        for (int j = 0; j < 4; ++j) {
+            const int ib = 1 + hg * kSymGroup + j * kWave + lane;
+            const float4 v = ib < sp.M ? J32[ib] : make_float4(-1.0e18f, 0.0f, 0.0f, 0.0f);
+            float qx = v.x, qy = v.y, qz = v.z, qm = v.w;
+            float qt = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
+            float fx[4] = {0.f, 0.f, 0.f, 0.f}, fy[4] = {0.f, 0.f, 0.f, 0.f}, fz[4] = {0.f, 0.f, 0.f, 0.f};
+            float gx = 0.f, gy = 0.f, gz = 0.f;
+            uint64_t wave_near = 0;   // lanes that met a close pair in this packet's pass
+            // the FP32 pass; the diagonal test is resolved at compile time so
+            // that the off-diagonal loop has no branch
+            auto fp32_pass = [&](auto diag_c) {
+            constexpr bool D = decltype(diag_c)::value;
+            for (int k = 0; k < kWave; ++k) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (D && !active(true, t, j, k)) continue;
+                    const float dx = qx - px[t], dy = qy - py[t], dz = qz - pz[t];
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    const float y0 = __builtin_amdgcn_rsqf(SOFT ? r2 + eps2 : r2);
+                    const bool near = r2 < th[t] + qt;
+                    wave_near |= __ballot(near);
+                    const float y0m = near ? 0.0f : y0;
+                    const float wv = y0m * (y0 * y0);
+                    const float fa = qm * wv, fb = pm[t] * wv;
+                    fx[t] = __builtin_fmaf(fa, dx, fx[t]);
+                    fy[t] = __builtin_fmaf(fa, dy, fy[t]);
+                    fz[t] = __builtin_fmaf(fa, dz, fz[t]);
+                    gx = __builtin_fmaf(-fb, dx, gx);
+                    gy = __builtin_fmaf(-fb, dy, gy);
+                    gz = __builtin_fmaf(-fb, dz, gz);
+                }
+                qx = rot_lane_f(qx); qy = rot_lane_f(qy); qz = rot_lane_f(qz); qm = rot_lane_f(qm);
+                qt = rot_lane_f(qt);
+                gx = rot_lane_f(gx); gy = rot_lane_f(gy); gz = rot_lane_f(gz);
+            }
+            };
+            if (diag) fp32_pass(std::true_type{}); else fp32_pass(std::false_type{});
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                ax[t] += (double)fx[t];
+                ay[t] += (double)fy[t];
+                az[t] += (double)fz[t];
+            }
+            double cx = (double)gx, cy = (double)gy, cz = (double)gz;
+            if (wave_near) {
+                // replay the packet's steps for the close pairs, in FP64
+                qx = v.x; qy = v.y; qz = v.z;
+                qt = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
+                double hx = 0.0, hy = 0.0, hz = 0.0;
+                for (int k = 0; k < kWave; ++k) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        if (!active(diag, t, j, k)) continue;
+                        const float dx = qx - px[t], dy = qy - py[t], dz = qz - pz[t];
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        if (r2 < th[t] + qt) {
+                            const double4 A = J[1 + ag * kSymGroup + t * kWave + lane];
+                            const double4 B = J[1 + hg * kSymGroup + j * kWave + ((lane + k) & (kWave - 1))];
+                            const double ex = B.x - A.x, ey = B.y - A.y, ez = B.z - A.z;
+                            const double s2 = ex * ex + ey * ey + ez * ez;
+                            const double rs = SOFT ? s2 + eps2d : s2;
+                            const double y0 = __builtin_amdgcn_rsq(rs);
+                            const double tt = y0 * y0;
+                            const double e = __builtin_fma(-rs, tt, 1.0);
+                            const double y3 = tt * y0;
+                            const double wv = __builtin_fma(y3, e * __builtin_fma(c1875, e, c15), y3);
+                            const double fa = B.w * wv, fb = A.w * wv;
+                            ax[t] = __builtin_fma(fa, ex, ax[t]);
+                            ay[t] = __builtin_fma(fa, ey, ay[t]);
+                            az[t] = __builtin_fma(fa, ez, az[t]);
+                            hx = __builtin_fma(-fb, ex, hx);
+                            hy = __builtin_fma(-fb, ey, hy);
+                            hz = __builtin_fma(-fb, ez, hz);
+                        }
+                    }
+                    qx = rot_lane_f(qx); qy = rot_lane_f(qy); qz = rot_lane_f(qz); qt = rot_lane_f(qt);
+                    hx = rot_lane(hx); hy = rot_lane(hy); hz = rot_lane(hz);
+                }
+                cx += hx; cy += hy; cz += hz;
+            }
+            bs[j][0] += cx;
+            bs[j][1] += cy;
+            bs[j][2] += cz;
+        }
+        double *r = apart + it * (3 * kSymGroup);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            r[t * kWave + lane] = ax[t];
+            r[kSymGroup + t * kWave + lane] = ay[t];
+            r[2 * kSymGroup + t * kWave + lane] = az[t];
+        }
+    }
+    if (hcur >= 0) flush_col();
+}
+
 }  // namespace
+
+void launch_pairs_sym_mixed(hipStream_t st, const SymPlan &sp, const float4 *J32, const double4 *J, double *apart,
+                            double *bpart, double eps2)
+{
+    if (sp.M <= 2) return;
+    const int blocks = (sp.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
+    const float nf2 = kNearFrac * kNearFrac;
+    if (eps2 != 0.0)
+        hipLaunchKernelGGL(k_pairs_sym_mixed<true>, dim3(blocks), dim3(kBlock), 0, st, sp, J32, J, apart, bpart,
+                           (float)eps2, eps2, nf2, 1.5, 1.875);
+    else
+        hipLaunchKernelGGL(k_pairs_sym_mixed<false>, dim3(blocks), dim3(kBlock), 0, st, sp, J32, J, apart, bpart,
+                           (float)eps2, eps2, nf2, 1.5, 1.875);
+}
 
 void launch_pairs_sym(hipStream_t st, const SymPlan &sp, const double4 *J, double *apart, double *bpart,
                       double eps2)

@@ -132,6 +132,9 @@ void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double
 // in apart, column-side records in bpart.
 void launch_pairs_sym(hipStream_t st, const SymPlan &sp, const double4 *J, double *apart, double *bpart,
                       double eps2);
+// Its mixed-precision form (FP32 terms from J32, close pairs in FP64 from J).
+void launch_pairs_sym_mixed(hipStream_t st, const SymPlan &sp, const float4 *J32, const double4 *J, double *apart,
+                            double *bpart, double eps2);
 void launch_prep(hipStream_t st, int n, int np, const double *state, const double *gm,
                  double4 *J, float4 *J32, double *ind_out, int barycentric, int src_end_a,
                  int src_end_b);

@@ -313,6 +313,8 @@ __device__ bool gather_pairs(const EpiArgs &a, int i, double &px, double &py, do
         // h >= g, then the column records of the waves that cover column g
         const SymPlan &sp = a.sym;
         const int g = (i - 1) / kSymGroup, r = (i - 1) - g * kSymGroup;
+        // row records: independent loads, unrolled so that many are in flight
+#pragma unroll 8
         for (int h = g; h < sp.ngroups; ++h) {
             const double *q = a.apart + (sym_col_start(h) + g) * (3 * kSymGroup);
             px += q[r];
