@@ -463,9 +463,12 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
 #pragma unroll
             for (int c = 0; c < 3; ++c) q[c * kSymGroup + t * kWave + lane] = bs[t][c];
     };
-    auto active = [&](bool diag, int t, int j, int k) {
-        return !diag || t < j || (t == j && k >= 1 && (k < 32 || (k == 32 && lane < 32)));
-    };
+    // diagonal items: row tile t meets packet j for t < j, and for t == j at
+    // steps 1..32 (wave-uniform tests); at step 32 of t == j only lanes 0..31
+    // take the pair, which is masked arithmetically, not branched on, so that
+    // every lane reaches the packet rotations
+    auto skip = [&](bool diag, int t, int j, int k) { return diag && (t > j || (t == j && (k < 1 || k > 32))); };
+    auto masked = [&](bool diag, int t, int j, int k) { return diag && t == j && k == 32 && lane >= 32; };
 
     for (int64_t it = lo; it < hi; ++it) {
         int ag, hg;
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
             float qt = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
             float fx[4] = {0.f, 0.f, 0.f, 0.f}, fy[4] = {0.f, 0.f, 0.f, 0.f}, fz[4] = {0.f, 0.f, 0.f, 0.f};
             float gx = 0.f, gy = 0.f, gz = 0.f;
-            uint64_t wave_near = 0;   // lanes that met a close pair in this packet's pass
+            bool lane_near = false;   // this lane met a close pair in this packet's pass
             // the FP32 pass; the diagonal test is resolved at compile time so
             // that the off-diagonal loop has no branch
             auto fp32_pass = [&](auto diag_c) {
@@ -502,13 +505,14 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
             for (int k = 0; k < kWave; ++k) {
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    if (D && !active(true, t, j, k)) continue;
+                    if (D && skip(true, t, j, k)) continue;
+                    const bool off = D && masked(true, t, j, k);
                     const float dx = qx - px[t], dy = qy - py[t], dz = qz - pz[t];
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                     const float y0 = __builtin_amdgcn_rsqf(SOFT ? r2 + eps2 : r2);
-                    const bool near = r2 < th[t] + qt;
-                    wave_near |= __ballot(near);
-                    const float y0m = near ? 0.0f : y0;
+                    const bool near = !off && r2 < th[t] + qt;
+                    lane_near |= near;
+                    const float y0m = (near || off) ? 0.0f : y0;
                     const float wv = y0m * (y0 * y0);
                     const float fa = qm * wv, fb = pm[t] * wv;
                     fx[t] = __builtin_fmaf(fa, dx, fx[t]);
@@ -531,7 +535,7 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
                 az[t] += (double)fz[t];
             }
             double cx = (double)gx, cy = (double)gy, cz = (double)gz;
-            if (wave_near) {
+            if (__any(lane_near)) {
                 // replay the packet's steps for the close pairs, in FP64
                 qx = v.x; qy = v.y; qz = v.z;
                 qt = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
@@ -539,10 +543,10 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
                 for (int k = 0; k < kWave; ++k) {
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
-                        if (!active(diag, t, j, k)) continue;
+                        if (skip(diag, t, j, k)) continue;
                         const float dx = qx - px[t], dy = qy - py[t], dz = qz - pz[t];
                         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        if (r2 < th[t] + qt) {
+                        if (!masked(diag, t, j, k) && r2 < th[t] + qt) {
                             const double4 A = J[1 + ag * kSymGroup + t * kWave + lane];
                             const double4 B = J[1 + hg * kSymGroup + j * kWave + ((lane + k) & (kWave - 1))];
                             const double ex = B.x - A.x, ey = B.y - A.y, ez = B.z - A.z;
