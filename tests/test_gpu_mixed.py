@@ -63,16 +63,27 @@ def test_mixed_full_size_against_openmp_oracle():
     assert tot.max() <= 1e-7
 
 
-def test_mixed_rkf78_trajectory_close_to_fp64():
+@pytest.mark.parametrize("integrator", [S.RUNGE_KUTTA_FEHLBERG78, S.DORMAND_PRINCE])
+def test_mixed_trajectory_close_to_fp64(integrator):
+    """Replays the FP64 Driver's accepted step sizes: the mixed Driver must
+    accept every one of them at its first attempt (its error estimate stays
+    close to the FP64 one), and the positions stay within 1e-9 AU.  At 2,049
+    bodies the pair sums of all stages but the last use the symmetric kernel,
+    the last and its error-estimate partner the one-directional kernel."""
     scn = S.make_c3(n=2048)
     f = runtime.Context(scn)
     m = runtime.Context(scn, precision=runtime.MIXED)
     f.set_state(scn.y0); m.set_state(scn.y0)
     tf = _abi.timeline(0.0, scn.h_next, 1e12)
+    m.profile(True)
     for k in range(60):
-        assert f.driver(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, tf)[0] == 0
+        assert f.driver(integrator, 1e-10, tf)[0] == 0
         tm = _abi.timeline(tf.time - tf.hDid, tf.hDid, 1e12)
-        assert m.driver(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, tm)[0] == 0
+        assert m.driver(integrator, 1e-10, tm)[0] == 0
+        assert tm.hDid == tf.hDid, f"step {k}: mixed Driver took {tm.hDid} instead of {tf.hDid}"
+    prof = m.profile(False)
     d = np.abs(f.get_array(runtime.Y0)[:, :3] - m.get_array(runtime.Y0)[:, :3]).max()
-    print(f"  mixed vs FP64 after 60 replayed steps (t = {tf.time:.1f} d): {d:.2e} AU", flush=True)
+    print(f"  mixed vs FP64 after 60 replayed steps (t = {tf.time:.1f} d): {d:.2e} AU; "
+          f"symmetric launches {prof.pair_sym_launches}", flush=True)
+    assert prof.pair_sym_launches > 0
     assert d <= 1e-9

@@ -141,3 +141,25 @@ def test_sym_mixed_accuracy(n):
     print(f"  n={scn.n} mixed symmetric: total rel err max {err.max():.2e}, self-gravity max {sg.max():.2e} "
           f"median {np.median(sg):.2e}", flush=True)
     assert err.max() <= 1e-7
+
+
+def test_sym_mixed_close_pairs_in_every_position():
+    """Close pairs (evaluated in FP64 by the replay pass) placed where the
+    symmetric kernel meets them in each way: the diagonal half step (lanes 0
+    and 32 of one tile, step 32), another diagonal step, two tiles of one
+    group, and two groups (an off-diagonal item).  Regression test for
+    packet rotations under a partial EXEC mask."""
+    scn = S.make_c3(n=4096)
+    y0 = scn.y0.copy()
+    pairs = [(65, 97), (130, 131), (3, 200), (10, 2000), (1500, 3900)]
+    for a, b in pairs:
+        y0[b, :3] = y0[a, :3] + np.array([7.0e-4, -3.0e-4, 1.0e-4])
+    scn.y0 = y0
+    ctx = runtime.Context(scn, precision=runtime.MIXED)
+    ctx.set_side_outputs(False)
+    a_m = ctx.compute(0.0, y0)
+    ctx.close()
+    b = oracle_compute(scn)
+    err = rel_err(a_m[:, 3:], b[:, 3:])
+    print(f"  close pairs {pairs}: max rel err {err.max():.2e} at body {int(err.argmax())}", flush=True)
+    assert err.max() <= 1e-7
