@@ -300,18 +300,27 @@ void launch_mixed(hipStream_t st, const PairPlan &plan, const float4 *J32, const
 // ---- symmetric pair sum over the mutual block ------------------------------
 // One wave per range of group-pair items (sym_plan.h).  The wave holds the
 // 256 bodies of the row group, four per lane, and streams the column group
-// as four packets of 64 bodies, one per lane, that rotate one lane per step
-// (DPP wave_rol:1): after 64 steps every row body has met every packet body
-// and the packet, which carries its own force sums, is back on its lane.
-// Each step evaluates four pairs per lane and forms both accelerations from
-// each, so a pair of the block costs one reciprocal square root and 20 FP64
-// operations for two interactions, plus the packet's 14 rotated dwords per
-// step (3.5 per pair).  In a diagonal item (row == column) the row tile t
-// meets packet j only for t < j, and for t == j at steps 1..32 (step 32 on
-// lanes 0..31): each unordered pair once, no self pair.
+// as four packets of 64 bodies.  At step k (0..63) lane l pairs its four
+// row bodies with packet body (l + k) mod 64: the packet's positions and
+// k^2 m are read from the wave's own LDS slice, where the packet is stored
+// twice in a row so that entry l + k needs no wrap, and the packet's force
+// sums travel with it, rotating one lane per step (DPP wave_rol:1); after
+// 64 steps every row body has met every packet body and the sums are back
+// on their lanes.  Each step evaluates four pairs per lane and forms both
+// accelerations from each, so a pair of the block costs one reciprocal
+// square root and 20 FP64 operations for two interactions, plus 6 rotated
+// dwords and two 16-byte LDS reads per step of four pairs.  In a diagonal
+// item (row == column) the row tile t meets packet j only for t < j, and
+// for t == j at steps 1..32 (step 32 on lanes 0..31): each unordered pair
+// once, no self pair.
 // Bodies past the end of the block are far sentinels of zero mass at
 // +1e30 AU (row side) and -1e30 AU (column side), so that no pair has zero
 // distance; their sums are never read.
+//
+// The LDS slices are private to their waves, and the four waves of a block
+// run different item ranges, so there is no block barrier: a wave's LDS
+// operations execute in order, and a wavefront-scope fence keeps the
+// compiler from moving a packet's reads across the stores of the next.
 
 __device__ __forceinline__ double rot_lane(double v)
 {
@@ -349,8 +358,10 @@ __global__ __launch_bounds__(256) void k_pairs_sym_fp64(SymPlan sp, const double
                                                         double *__restrict__ apart, double *__restrict__ bpart,
                                                         double eps2, double c15, double c1875)
 {
+    __shared__ double4 pk_lds[kBlock / kWave][2 * kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int w = blockIdx.x * (kBlock / kWave) + wib;
     if (w >= sp.nwaves) return;
     const int64_t lo = sym_wave_lo(sp, w), hi = sym_wave_lo(sp, w + 1);
     if (lo >= hi) return;
@@ -358,6 +369,7 @@ __global__ __launch_bounds__(256) void k_pairs_sym_fp64(SymPlan sp, const double
     sym_decode(lo, a0, h0);
     int hcur = -1;
     double bs[4][3];                         // column-side sums of the current column group
+    double4 *pk = pk_lds[wib];
 
     auto flush_col = [&]() {
         double *q = bpart + ((int64_t)w * sp.slots_per_wave + (hcur - h0)) * (3 * kSymGroup);
@@ -387,30 +399,38 @@ __global__ __launch_bounds__(256) void k_pairs_sym_fp64(SymPlan sp, const double
         const bool diag = (ag == hg);
         for (int j = 0; j < 4; ++j) {
             const int ib = 1 + hg * kSymGroup + j * kWave + lane;
-            double4 q = ib < sp.M ? J[ib] : make_double4(-1.0e30, 0.0, 0.0, 0.0);
+            const double4 q0 = ib < sp.M ? J[ib] : make_double4(-1.0e30, 0.0, 0.0, 0.0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            pk[lane] = q0;
+            pk[lane + kWave] = q0;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
             double gx = 0.0, gy = 0.0, gz = 0.0;
-            if (!diag) {
-                for (int k = 0; k < kWave; ++k) {
+            // one step: the four row bodies against packet body q, then the
+            // packet's sums move one lane on
+            auto step = [&](const double4 &q, int k, auto diag_c) {
+                constexpr bool D = decltype(diag_c)::value;
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                for (int t = 0; t < 4; ++t) {
+                    if (!D || t < j || (t == j && k >= 1 && (k < 32 || (k == 32 && lane < 32))))
                         sym_pair<SOFT>(px[t], py[t], pz[t], pm[t], q, eps2, c15, c1875, fx[t], fy[t], fz[t],
                                        gx, gy, gz);
-                    q.x = rot_lane(q.x); q.y = rot_lane(q.y); q.z = rot_lane(q.z); q.w = rot_lane(q.w);
-                    gx = rot_lane(gx); gy = rot_lane(gy); gz = rot_lane(gz);
                 }
-            } else {
-                for (int k = 0; k < kWave; ++k) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const bool on = t < j || (t == j && k >= 1 && (k < 32 || (k == 32 && lane < 32)));
-                        if (on)
-                            sym_pair<SOFT>(px[t], py[t], pz[t], pm[t], q, eps2, c15, c1875, fx[t], fy[t],
-                                           fz[t], gx, gy, gz);
-                    }
-                    q.x = rot_lane(q.x); q.y = rot_lane(q.y); q.z = rot_lane(q.z); q.w = rot_lane(q.w);
-                    gx = rot_lane(gx); gy = rot_lane(gy); gz = rot_lane(gz);
+                gx = rot_lane(gx); gy = rot_lane(gy); gz = rot_lane(gz);
+            };
+            // two steps per iteration, each step's packet body read one step
+            // ahead (entry lane + k + 2 <= 127 stays in the slice)
+            auto steps = [&](auto diag_c) {
+                double4 qa = pk[lane];
+                for (int k = 0; k < kWave; k += 2) {
+                    const double4 qb = pk[lane + k + 1];
+                    step(qa, k, diag_c);
+                    qa = pk[lane + k + 2];
+                    step(qb, k + 1, diag_c);
                 }
-            }
+            };
+            if (diag) steps(std::true_type{}); else steps(std::false_type{});
             bs[j][0] += gx;
             bs[j][1] += gy;
             bs[j][2] += gz;
@@ -433,21 +453,30 @@ __device__ __forceinline__ float rot_lane_f(float v)
 
 // Mixed-precision form of k_pairs_sym_fp64: FP32 pair terms from J32 (and
 // FP32 sums over the 256 bodies an item pairs with each body, folded into
-// FP64), with the close pairs of k_pairs_mixed evaluated in FP64.  A pair
-// is close when |d|^2 < near_frac2 (|r_a|^2 + |r_b|^2) (same decision for
-// both bodies); the FP32 pass leaves it out and flags its lane, and after
-// the 64 steps of a packet a wave with a flagged lane replays the steps,
-// adding the FP64 terms of its close pairs (positions from J) to FP64 sums
-// that travel with the packet.  The packet rotates 8 dwords per step:
-// x, y, z, k^2 m, its close-pair threshold and its three FP32 sums.
+// FP64), with the close pairs evaluated in FP64.  A pair is close when
+// |d|^2 < near_frac2 |r_a|^2, r_a the row body (one decision for both
+// bodies; a close pair has |r_b| = |r_a| to within |d|, so this is the
+// one-directional kernel's test, |d| < near_frac |r_i|, from either side).
+// The softening is folded into the squared distance (rs = eps2 + dx^2 +
+// dy^2 + dz^2) and into the threshold (th = eps2 + near_frac2 |r_a|^2), so
+// the test is one compare.  The FP32 pass leaves a close pair out and
+// records it in a wave mask (a ballot, on the scalar unit); after the 64
+// steps of a packet a wave with a close pair replays the steps, recomputing
+// the same rs and the same test, and adds the FP64 terms of its close pairs
+// (positions from J) to FP64 sums that travel with the packet.  As in the
+// FP64 kernel the packet's (x, y, z, k^2 m) are read from the wave's LDS
+// slice, one 16-byte read per step, and only its three FP32 sums rotate.
+// Per pair: 18 FP32 VALU operations and one v_rsq_f32.
 template <bool SOFT>
 __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float4 *__restrict__ J32,
                                                          const double4 *__restrict__ J, double *__restrict__ apart,
                                                          double *__restrict__ bpart, float eps2, double eps2d,
                                                          float near_frac2, double c15, double c1875)
 {
+    __shared__ float4 pk_lds[kBlock / kWave][2 * kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int w = blockIdx.x * (kBlock / kWave) + wib;
     if (w >= sp.nwaves) return;
     const int64_t lo = sym_wave_lo(sp, w), hi = sym_wave_lo(sp, w + 1);
     if (lo >= hi) return;
@@ -455,6 +484,8 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
     sym_decode(lo, a0, h0);
     int hcur = -1;
     double bs[4][3];
+    const float e2 = SOFT ? eps2 : 0.0f;
+    float4 *pk = pk_lds[wib];
 
     auto flush_col = [&]() {
         double *q = bpart + ((int64_t)w * sp.slots_per_wave + (hcur - h0)) * (3 * kSymGroup);
@@ -466,7 +497,8 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
     // diagonal items: row tile t meets packet j for t < j, and for t == j at
     // steps 1..32 (wave-uniform tests); at step 32 of t == j only lanes 0..31
     // take the pair, which is masked arithmetically, not branched on, so that
-    // every lane reaches the packet rotations
+    // every lane reaches the packet rotations and the FP32 pass has no
+    // divergent branch (its close-pair mask stays wave-uniform)
     auto skip = [&](bool diag, int t, int j, int k) { return diag && (t > j || (t == j && (k < 1 || k > 32))); };
     auto masked = [&](bool diag, int t, int j, int k) { return diag && t == j && k == 32 && lane >= 32; };
 
@@ -486,35 +518,38 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
             const int ia = 1 + ag * kSymGroup + t * kWave + lane;
             const float4 v = ia < sp.M ? J32[ia] : make_float4(1.0e18f, 0.0f, 0.0f, 0.0f);
             px[t] = v.x; py[t] = v.y; pz[t] = v.z; pm[t] = v.w;
-            th[t] = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
+            th[t] = __builtin_fmaf(near_frac2, __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x)), e2);
             ax[t] = ay[t] = az[t] = 0.0;
         }
         const bool diag = (ag == hg);
         for (int j = 0; j < 4; ++j) {
             const int ib = 1 + hg * kSymGroup + j * kWave + lane;
             const float4 v = ib < sp.M ? J32[ib] : make_float4(-1.0e18f, 0.0f, 0.0f, 0.0f);
-            float qx = v.x, qy = v.y, qz = v.z, qm = v.w;
-            float qt = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            pk[lane] = v;
+            pk[lane + kWave] = v;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
             float fx[4] = {0.f, 0.f, 0.f, 0.f}, fy[4] = {0.f, 0.f, 0.f, 0.f}, fz[4] = {0.f, 0.f, 0.f, 0.f};
             float gx = 0.f, gy = 0.f, gz = 0.f;
-            bool lane_near = false;   // this lane met a close pair in this packet's pass
-            // the FP32 pass; the diagonal test is resolved at compile time so
-            // that the off-diagonal loop has no branch
-            auto fp32_pass = [&](auto diag_c) {
-            constexpr bool D = decltype(diag_c)::value;
-            for (int k = 0; k < kWave; ++k) {
+            uint64_t wave_far = ~0ull;   // lanes that met no close pair in this packet's pass
+            // one FP32 step against packet body q; the diagonal test is
+            // resolved at compile time so that the off-diagonal loop has no branch
+            auto step = [&](const float4 &q, int k, auto diag_c) {
+                constexpr bool D = decltype(diag_c)::value;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     if (D && skip(true, t, j, k)) continue;
                     const bool off = D && masked(true, t, j, k);
-                    const float dx = qx - px[t], dy = qy - py[t], dz = qz - pz[t];
-                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    const float y0 = __builtin_amdgcn_rsqf(SOFT ? r2 + eps2 : r2);
-                    const bool near = !off && r2 < th[t] + qt;
-                    lane_near |= near;
-                    const float y0m = (near || off) ? 0.0f : y0;
+                    const float dx = q.x - px[t], dy = q.y - py[t], dz = q.z - pz[t];
+                    const float rs = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, e2)));
+                    const float y0 = __builtin_amdgcn_rsqf(rs);
+                    const bool far = !(rs < th[t]);
+                    wave_far &= __ballot(far);
+                    const float y0m = (far && !off) ? y0 : 0.0f;
                     const float wv = y0m * (y0 * y0);
-                    const float fa = qm * wv, fb = pm[t] * wv;
+                    const float fa = q.w * wv, fb = pm[t] * wv;
                     fx[t] = __builtin_fmaf(fa, dx, fx[t]);
                     fy[t] = __builtin_fmaf(fa, dy, fy[t]);
                     fz[t] = __builtin_fmaf(fa, dz, fz[t]);
@@ -522,10 +557,18 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
                     gy = __builtin_fmaf(-fb, dy, gy);
                     gz = __builtin_fmaf(-fb, dz, gz);
                 }
-                qx = rot_lane_f(qx); qy = rot_lane_f(qy); qz = rot_lane_f(qz); qm = rot_lane_f(qm);
-                qt = rot_lane_f(qt);
                 gx = rot_lane_f(gx); gy = rot_lane_f(gy); gz = rot_lane_f(gz);
-            }
+            };
+            // two steps per iteration, each packet body read one step ahead
+            // (entry lane + k + 2 <= 127 stays in the slice)
+            auto fp32_pass = [&](auto diag_c) {
+                float4 qa = pk[lane];
+                for (int k = 0; k < kWave; k += 2) {
+                    const float4 qb = pk[lane + k + 1];
+                    step(qa, k, diag_c);
+                    qa = pk[lane + k + 2];
+                    step(qb, k + 1, diag_c);
+                }
             };
             if (diag) fp32_pass(std::true_type{}); else fp32_pass(std::false_type{});
 #pragma unroll
@@ -535,26 +578,26 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
                 az[t] += (double)fz[t];
             }
             double cx = (double)gx, cy = (double)gy, cz = (double)gz;
-            if (__any(lane_near)) {
+            if (wave_far != ~0ull) {
                 // replay the packet's steps for the close pairs, in FP64
-                qx = v.x; qy = v.y; qz = v.z;
-                qt = near_frac2 * __builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x));
                 double hx = 0.0, hy = 0.0, hz = 0.0;
                 for (int k = 0; k < kWave; ++k) {
+                    const float4 q = pk[lane + k];
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         if (skip(diag, t, j, k)) continue;
-                        const float dx = qx - px[t], dy = qy - py[t], dz = qz - pz[t];
-                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        if (!masked(diag, t, j, k) && r2 < th[t] + qt) {
+                        const float dx = q.x - px[t], dy = q.y - py[t], dz = q.z - pz[t];
+                        const float rs =
+                            __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, e2)));
+                        if (!masked(diag, t, j, k) && rs < th[t]) {
                             const double4 A = J[1 + ag * kSymGroup + t * kWave + lane];
                             const double4 B = J[1 + hg * kSymGroup + j * kWave + ((lane + k) & (kWave - 1))];
                             const double ex = B.x - A.x, ey = B.y - A.y, ez = B.z - A.z;
                             const double s2 = ex * ex + ey * ey + ez * ez;
-                            const double rs = SOFT ? s2 + eps2d : s2;
-                            const double y0 = __builtin_amdgcn_rsq(rs);
+                            const double r2d = SOFT ? s2 + eps2d : s2;
+                            const double y0 = __builtin_amdgcn_rsq(r2d);
                             const double tt = y0 * y0;
-                            const double e = __builtin_fma(-rs, tt, 1.0);
+                            const double e = __builtin_fma(-r2d, tt, 1.0);
                             const double y3 = tt * y0;
                             const double wv = __builtin_fma(y3, e * __builtin_fma(c1875, e, c15), y3);
                             const double fa = B.w * wv, fb = A.w * wv;
@@ -566,7 +609,6 @@ __global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float
                             hz = __builtin_fma(-fb, ez, hz);
                         }
                     }
-                    qx = rot_lane_f(qx); qy = rot_lane_f(qy); qz = rot_lane_f(qz); qt = rot_lane_f(qt);
                     hx = rot_lane(hx); hy = rot_lane(hy); hz = rot_lane(hz);
                 }
                 cx += hx; cy += hy; cz += hz;
