@@ -161,8 +161,9 @@ def traffic_probe(n: int):
     (FETCH_SIZE, then WRITE_SIZE) that give the pair kernel's HBM traffic:
     a plain device-to-device copy of known size (2 GiB read, 2 GiB
     written), whose counter readings calibrate the counters, then one
-    accepted RKF78 step of the benchmark workload (12 pair launches without
-    and one with the nearest-neighbour search)."""
+    accepted RKF78 step of the benchmark workload (per attempt, 12
+    launches of the symmetric pair kernel and one of the one-directional
+    kernel with the nearest-neighbour search)."""
     import torch
     from solaris_amd import _abi, runtime, scenarios as S
     src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
