@@ -20,6 +20,7 @@ namespace {
 struct Entry {
     sol_context *ctx = nullptr;
     int n = -1;
+    int rm3_cap = 0;          // length of the Acceleration::rm3 array this bridge allocated
     bool barycentric = false;
     bool has_gas = false;
     sol_gas_component gas{};
@@ -164,9 +165,19 @@ int FetchSideOutputs(Acceleration *acc, sol_context *ctx)
 {
     BodyData *bd = acc->bodyData;
     const int n = bd->nBodies.total;
-    if (!acc->rm3) {
-        acc->rm3 = new double[n];
-        std::memset(acc->rm3, 0, n * sizeof(double));
+    // Acceleration::rm3 is allocated on first use, as the reference does
+    // (Solaris/Acceleration.cpp:65-69).  The reference never grows it; the
+    // bridge reallocates it if a caller adds bodies, so that writing n
+    // values stays in bounds.
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        Entry &e = g_table[acc];
+        if (!acc->rm3 || e.rm3_cap < n) {
+            delete[] acc->rm3;
+            acc->rm3 = new double[n];
+            std::memset(acc->rm3, 0, n * sizeof(double));
+            e.rm3_cap = n;
+        }
     }
     std::vector<int32_t> nn(n), mig(n);
     if (sol_get_side_outputs(ctx, acc->rm3, nn.data(), bd->distanceOfNN, mig.data()))
