@@ -353,6 +353,8 @@ __device__ __forceinline__ void sym_pair(double x, double y, double z, double gm
     gz = __builtin_fma(-fb, dz, gz);
 }
 
+// 246 VGPRs, two waves per SIMD: bounding it to three (168 VGPRs, four
+// values spilled outside the step loop) measured 1.1% slower at C3
 template <bool SOFT>
 __global__ __launch_bounds__(256) void k_pairs_sym_fp64(SymPlan sp, const double4 *__restrict__ J,
                                                         double *__restrict__ apart, double *__restrict__ bpart,
@@ -467,8 +469,10 @@ __device__ __forceinline__ float rot_lane_f(float v)
 // FP64 kernel the packet's (x, y, z, k^2 m) are read from the wave's LDS
 // slice, one 16-byte read per step, and only its three FP32 sums rotate.
 // Per pair: 18 FP32 VALU operations and one v_rsq_f32.
+// At least four waves per SIMD (128 VGPRs; six values spilled outside the
+// step loop): 1.5% faster at C3 than the unbounded three waves
 template <bool SOFT>
-__global__ __launch_bounds__(256) void k_pairs_sym_mixed(SymPlan sp, const float4 *__restrict__ J32,
+__global__ __launch_bounds__(256, 4) void k_pairs_sym_mixed(SymPlan sp, const float4 *__restrict__ J32,
                                                          const double4 *__restrict__ J, double *__restrict__ apart,
                                                          double *__restrict__ bpart, float eps2, double eps2d,
                                                          float near_frac2, double c15, double c1875)
