@@ -137,7 +137,13 @@ typedef struct sol_timeline {
     double  lastSave;
     double  lastNSteps;
     int32_t millenium;
-    int32_t reserved;
+    /* Accepted steps since the start of the run modulo CheckForSM (100):
+     * Counter::succededStep % Constants::CheckForSM of the caller loop
+     * (Solaris/Simulator.cpp:154-159, Solaris/Constants.h:16), which times
+     * the small-number flush.  Start a run with 0; sol_integrate and
+     * sol_integrate_until keep it, so that a run split over many calls
+     * flushes at the same steps as one call. */
+    int32_t flush_count;
 } sol_timeline;
 
 /* Counters filled by sol_integrate(). */
@@ -240,11 +246,47 @@ int sol_error_max(sol_context *ctx, int32_t integrator, double epsilon,
 /* The Simulator::Integrate loop (Solaris/Simulator.cpp:121-178) without
  * events and file output: Driver calls, the step clipping of
  * Simulator::DecisionMaking (Solaris/Simulator.cpp:181-248) and the
- * |y| < 1e-50 flush every 100 steps (:159-162), until the end of the
- * time line or max_steps accepted steps (max_steps <= 0: no limit).
- * stats may be NULL. */
+ * |y| < 1e-50 flush at every 100th accepted step (:159-162), until the end
+ * of the time line or max_steps accepted steps (max_steps <= 0: no limit).
+ * stats may be NULL.  The same as sol_integrate_until with stop_mask 0. */
 int sol_integrate(sol_context *ctx, int32_t integrator, double epsilon,
                   sol_timeline *tl, int64_t max_steps, sol_run_stats *stats);
+
+/* Thresholds of the caller loop's event checks, Simulator::CheckEvent
+ * (Solaris/Simulator.cpp:621-735), from the run's Settings (ejection,
+ * hitCentrum, collision->factor; :626-629, :688).  0 turns a check off.
+ * No counterpart as a call: the reference reads them from its settings. */
+typedef struct sol_event_settings {
+    double ejection;          /* AU: body i >= 1 with rm3[i] < 1/ejection^3 */
+    double hit_centrum;       /* AU: body i >= 1 with rm3[i] > 1/hit_centrum^3 */
+    double collision_factor;  /* body i whose nearest neighbour j has
+                                 factor * (radius[i] + radius[j]) > distanceOfNN[i] */
+} sol_event_settings;
+int sol_set_events(sol_context *ctx, const sol_event_settings *events);
+
+/* Reasons sol_integrate_until returns (bits of *reason). */
+#define SOL_STOP_END     1  /* the time line's length was reached (DecisionMaking's stop) */
+#define SOL_STOP_OUTPUT  2  /* an output is due: DecisionMaking would save the phases and
+                               integrals now (Solaris/Simulator.cpp:236-240) */
+#define SOL_STOP_EVENT   4  /* CheckEvent would find an ejection, hit-centrum or collision */
+#define SOL_STOP_STEPS   8  /* max_steps accepted steps */
+
+/* sol_integrate that returns after the first accepted step at which the
+ * caller loop has host work: with SOL_STOP_OUTPUT in stop_mask when an
+ * output is due, with SOL_STOP_EVENT when an event threshold set by
+ * sol_set_events is crossed (on the last stage's rm3 and nearest-neighbour
+ * outputs, as CheckEvent reads them), and always at the end of the time
+ * line or after max_steps.  *reason receives every reason that holds for
+ * that step.  On return the time line is as DecisionMaking leaves it after
+ * the step (clipped hNext, lastSave reset if an output was due), device Y0
+ * holds the state to save, and sol_get_side_outputs gives the arrays
+ * CheckEvent reads.  A caller that removes or merges bodies creates a
+ * context for the new body set and continues with the same time line.
+ * No counterpart in the reference, whose loop returns to the host after
+ * every step; this lets a run stay on the device between outputs. */
+int sol_integrate_until(sol_context *ctx, int32_t integrator, double epsilon,
+                        sol_timeline *tl, int64_t max_steps, int32_t stop_mask,
+                        int32_t *reason, sol_run_stats *stats);
 
 /* Calculate::Integrals on device Y0 (Solaris/Calculate.cpp:43-63):
  * out[16] = M, barycentre r and v, |r_bc|, |v_bc|, angular momentum and its

@@ -38,7 +38,14 @@ class TimeLine(C.Structure):
     _fields_ = [("start", C.c_double), ("length", C.c_double), ("output", C.c_double),
                 ("save", C.c_double), ("time", C.c_double), ("hNext", C.c_double),
                 ("hDid", C.c_double), ("elapsedTime", C.c_double), ("lastSave", C.c_double),
-                ("lastNSteps", C.c_double), ("millenium", C.c_int32), ("reserved", C.c_int32)]
+                ("lastNSteps", C.c_double), ("millenium", C.c_int32), ("flush_count", C.c_int32)]
+
+
+class EventSettings(C.Structure):
+    _fields_ = [("ejection", C.c_double), ("hit_centrum", C.c_double), ("collision_factor", C.c_double)]
+
+
+STOP_END, STOP_OUTPUT, STOP_EVENT, STOP_STEPS = 1, 2, 4, 8
 
 
 class RunStats(C.Structure):

@@ -509,7 +509,7 @@ bool use_resident(const sol_context *c)
 // max_steps steps) when clip is true.  Launches are bounded to about 0.2 s
 // of work each.
 int resident_run(sol_context *c, int ik, double eps, sol_timeline *tl, int64_t max_steps, bool clip, double *body_h,
-                 int64_t *steps_out, int64_t *attempts_out)
+                 int64_t *steps_out, int64_t *attempts_out, int32_t stop_mask = 0, int32_t *reason = nullptr)
 {
     ResSystem &S = *c->res_h;
     S = ResSystem{};
@@ -524,11 +524,15 @@ int resident_run(sol_context *c, int ik, double eps, sol_timeline *tl, int64_t m
     S.t = tl->time; S.h_next = tl->hNext; S.h_did = tl->hDid; S.eps = eps;
     S.elapsed = tl->elapsedTime; S.last_save = tl->lastSave; S.last_nsteps = tl->lastNSteps;
     S.length = tl->length; S.output = tl->output;
+    S.flush_count = tl->flush_count;
+    S.stop_mask = stop_mask & (SOL_STOP_OUTPUT | SOL_STOP_EVENT);
+    S.e3 = c->ev_e3; S.h3 = c->ev_h3; S.coll = c->ev_factor;
     HIPCHECK(c, hipMemcpyAsync(c->res_d, c->res_h, sizeof(ResSystem), hipMemcpyHostToDevice, c->stream));
     ResArrays A{};
     A.tot = c->np;
     A.y0 = c->y0; A.y = c->y; A.acc0 = c->acc0; A.yscale = c->yscale; A.err = c->error; A.fst = c->fst;
     A.mass = c->mass; A.rm3 = c->rm3; A.nnd = c->nnd; A.nni = c->nni;
+    A.radius = c->radius;
     const int block = resident_block_size(c->n);
     const double per_step = 13.0 * (double)c->n * (double)c->n / 4.0e9 + 2.0e-5;
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(20000, (int64_t)(0.2 / per_step)));
@@ -543,6 +547,13 @@ int resident_run(sol_context *c, int ik, double eps, sol_timeline *tl, int64_t m
     }
     tl->time = S.t; tl->hNext = S.h_next; tl->hDid = S.h_did;
     tl->elapsedTime = S.elapsed; tl->lastSave = S.last_save; tl->lastNSteps = S.last_nsteps;
+    if (clip) tl->flush_count = S.flush_count;
+    if (reason) {
+        // the last step's reasons (S.reason is written at every clipped step)
+        *reason = S.reason & (stop_mask | SOL_STOP_END);
+        if (S.status == RES_DONE) *reason |= SOL_STOP_END;
+        if (max_steps > 0 && S.steps >= max_steps) *reason |= SOL_STOP_STEPS;
+    }
     if (body_h) *body_h = S.body_h;
     if (steps_out) *steps_out += S.steps;
     if (attempts_out) *attempts_out += S.attempts;
@@ -983,9 +994,53 @@ int sol_error_max(sol_context *c, int32_t integrator, double eps, double *out)
 int sol_integrate(sol_context *c, int32_t integrator, double eps, sol_timeline *tl, int64_t max_steps,
                   sol_run_stats *stats)
 {
-    if (check_ctx(c) || !tl) return SOL_ERR_ARGUMENT;
+    int32_t reason = 0;
+    return sol_integrate_until(c, integrator, eps, tl, max_steps, 0, &reason, stats);
+}
+
+int sol_set_events(sol_context *c, const sol_event_settings *ev)
+{
+    if (check_ctx(c) || !ev) return SOL_ERR_ARGUMENT;
+    if (!(ev->ejection >= 0.0) || !(ev->hit_centrum >= 0.0) || !(ev->collision_factor >= 0.0))
+        return c->fail(SOL_ERR_ARGUMENT, "event thresholds must be >= 0 (0 turns a check off)");
+    // Simulator::CheckEvent's e3 and h3 (Solaris/Simulator.cpp:628-629)
+    c->ev_e3 = ev->ejection > 0.0 ? 1.0 / (ev->ejection * ev->ejection * ev->ejection) : 0.0;
+    c->ev_h3 = ev->hit_centrum > 0.0 ? 1.0 / (ev->hit_centrum * ev->hit_centrum * ev->hit_centrum) : 0.0;
+    c->ev_factor = ev->collision_factor;
+    return SOL_OK;
+}
+
+// True when CheckEvent would find an event on the side outputs of the last
+// evaluation (device check, one flag read back).
+static int events_now(sol_context *c, bool &event)
+{
+    event = false;
+    if (c->ev_e3 == 0.0 && c->ev_h3 == 0.0 && c->ev_factor == 0.0) return SOL_OK;
+    if (!c->ev_flag) {
+        int r = dalloc(c, &c->ev_flag, 1);
+        if (r) return r;
+    }
+    launch_event_check(c->stream, c->n, c->rm3, c->nni, c->nnd, c->radius, c->ev_e3, c->ev_h3, c->ev_factor,
+                       c->ev_flag);
+    HIPCHECK(c, hipGetLastError());
+    int32_t f = 0;
+    HIPCHECK(c, hipMemcpyAsync(&f, c->ev_flag, sizeof f, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    event = f != 0;
+    return SOL_OK;
+}
+
+int sol_integrate_until(sol_context *c, int32_t integrator, double eps, sol_timeline *tl, int64_t max_steps,
+                        int32_t stop_mask, int32_t *reason, sol_run_stats *stats)
+{
+    if (check_ctx(c) || !tl || !reason) return SOL_ERR_ARGUMENT;
+    *reason = 0;
     const int ik = integ_kind(integrator);
     if (ik < 0) return c->fail(SOL_ERR_UNSUPPORTED, "integrator %d is not provided", integrator);
+    if (stop_mask & ~(SOL_STOP_OUTPUT | SOL_STOP_EVENT | SOL_STOP_END | SOL_STOP_STEPS))
+        return c->fail(SOL_ERR_ARGUMENT, "unknown bits in stop_mask 0x%x", stop_mask);
+    if (tl->flush_count < 0 || tl->flush_count >= 100)
+        return c->fail(SOL_ERR_ARGUMENT, "timeline flush_count %d is not in [0, 100)", tl->flush_count);
     HIPCHECK(c, hipSetDevice(c->device));
     hipEvent_t e0, e1;
     HIPCHECK(c, hipEventCreate(&e0));
@@ -994,7 +1049,7 @@ int sol_integrate(sol_context *c, int32_t integrator, double eps, sol_timeline *
     int64_t steps = 0, attempts = 0;
     int r = SOL_OK;
     if (use_resident(c)) {
-        r = resident_run(c, ik, eps, tl, max_steps, true, nullptr, &steps, &attempts);
+        r = resident_run(c, ik, eps, tl, max_steps, true, nullptr, &steps, &attempts, stop_mask, reason);
     } else for (;;) {
         double bh = 0.0;
         if (ik == INTEG_RKF78) r = driver_rkf78(c, eps, tl, &bh, &attempts);
@@ -1002,22 +1057,35 @@ int sol_integrate(sol_context *c, int32_t integrator, double eps, sol_timeline *
         else r = driver_dp(c, eps, tl, &bh, &attempts);
         if (r) break;
         ++steps;
+        int32_t why = 0;
         // Simulator::DecisionMaking, Solaris/Simulator.cpp:183-245
         tl->elapsedTime += tl->hDid;
         tl->lastSave += tl->hDid;
         tl->lastNSteps += tl->hDid;
+        if (stop_mask & SOL_STOP_EVENT) {     // CheckEvent (:621-735) comes first
+            bool ev = false;
+            if ((r = events_now(c, ev))) break;
+            if (ev) why |= SOL_STOP_EVENT;
+        }
         const double actual = 1000.0 * 365.25 * tl->millenium + tl->time;
         bool stop = false;
         if (std::fabs(actual) >= std::fabs(tl->length)) stop = true;
         else if (c->n <= 1) stop = true;
         else {
             if (std::fabs(actual + tl->hNext) > std::fabs(tl->length)) tl->hNext = tl->length - actual;
-            if (std::fabs(tl->lastSave) >= std::fabs(tl->output)) tl->lastSave = 0.0;
+            if (std::fabs(tl->lastSave) >= std::fabs(tl->output)) {
+                tl->lastSave = 0.0;
+                why |= SOL_STOP_OUTPUT;
+            }
             if (std::fabs(tl->lastSave + tl->hNext) > std::fabs(tl->output)) tl->hNext = tl->output - tl->lastSave;
         }
-        if (steps % 100 == 0) launch_flush_small(c->stream, c->n, c->np, c->y, c->y0);   // :159-162
-        if (stop) break;
-        if (max_steps > 0 && steps >= max_steps) break;
+        // Counter::succededStep % CheckForSM (:154-162)
+        tl->flush_count = (tl->flush_count + 1) % 100;
+        if (tl->flush_count == 0) launch_flush_small(c->stream, c->n, c->np, c->y, c->y0);
+        why &= stop_mask;
+        if (stop) why |= SOL_STOP_END;
+        if (max_steps > 0 && steps >= max_steps) why |= SOL_STOP_STEPS;
+        if (why) { *reason = why; break; }
     }
     (void)hipEventRecord(e1, c->stream);
     (void)hipEventSynchronize(e1);

@@ -73,6 +73,10 @@ struct sol_context {
     size_t apart_cap = 0, bpart_cap = 0;
     bool sym_disabled = false;
     bool side_outputs = true;     // sol_compute tracks nearest neighbours (sol_set_side_outputs)
+    // event thresholds of sol_integrate_until (sol_set_events): 1/ejection^3,
+    // 1/hitCentrum^3 and the collision factor, each 0 when off
+    double ev_e3 = 0.0, ev_h3 = 0.0, ev_factor = 0.0;
+    int32_t *ev_flag = nullptr;   // device: any event at the last accepted step
     // recorded forward pass of the RK4 adjoint (adjoint.cpp)
     sol::AdjointState *adj = nullptr;
 

@@ -6,19 +6,27 @@
 
 namespace sol {
 
-enum { RES_RUNNING = 0, RES_DONE = 1, RES_UNDERFLOW = 2, RES_MAXITER = 3 };
+// RES_STOPPED: the step met a stop reason of stop_mask (SOL_STOP_OUTPUT,
+// SOL_STOP_EVENT), recorded in reason; the host resumes with RES_RUNNING.
+enum { RES_RUNNING = 0, RES_DONE = 1, RES_UNDERFLOW = 2, RES_MAXITER = 3, RES_STOPPED = 4 };
 constexpr int kResidentMaxBodies = 2000;   // LDS: 32 B per body
 
 // One system: its bodies are [base, base + n) of the concatenated arrays.
 // Time-line fields follow TimeLine (Solaris/TimeLine.h); clip switches on
-// the Simulator's step clipping and small-number flush.
+// the Simulator's step clipping and small-number flush, flush_count is the
+// caller loop's accepted-step count modulo 100.  With clip, stop_mask
+// (SOL_STOP_OUTPUT | SOL_STOP_EVENT) makes the system stop after a step at
+// which an output is due or an event threshold (e3 = 1/ejection^3,
+// h3 = 1/hitCentrum^3, coll = collision factor; 0 = off) is crossed.
 struct ResSystem {
     int32_t n, nm, end_a, end_b;
     int32_t base, barycentric, clip, status;
-    int32_t millenium, pad;
+    int32_t millenium, flush_count;
     double t, h_next, h_did, body_h, eps;
     double elapsed, last_save, last_nsteps, length, output;
     int64_t steps, attempts;
+    double e3, h3, coll;
+    int32_t stop_mask, reason;
 };
 
 // Structure-of-arrays storage over all systems: component c of body b at
@@ -27,6 +35,7 @@ struct ResArrays {
     int64_t tot;
     double *y0, *y, *acc0, *yscale, *err, *fst, *mass, *rm3, *nnd;
     int32_t *nni;
+    const double *radius;     // read by the collision check only
 };
 
 int resident_block_size(int n);

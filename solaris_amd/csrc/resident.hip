@@ -373,8 +373,24 @@ __global__ __launch_bounds__(1024) void k_resident(ResSystem *systems, ResArrays
                 at(A.y, c, k, i) = a;
             }
         __syncthreads();
-        bool stop = false;
+        // CheckEvent's thresholds on the last stage's side outputs
+        // (Solaris/Simulator.cpp:631-644, :688-695), when asked for
+        int event = 0;
+        if (S.clip && (S.stop_mask & SOL_STOP_EVENT) && (S.e3 > 0.0 || S.h3 > 0.0 || S.coll > 0.0)) {
+            for (int i = threadIdx.x; i < c.n; i += blockDim.x) {
+                const double r3 = A.rm3[c.base + i];
+                if (i >= 1 && S.e3 > 0.0 && r3 < S.e3) event = 1;
+                if (i >= 1 && S.h3 > 0.0 && r3 > S.h3) event = 1;
+                const int j = A.nni[c.base + i];
+                if (S.coll > 0.0 && j >= 0 && j < c.n &&
+                    S.coll * (A.radius[c.base + i] + A.radius[c.base + j]) > A.nnd[c.base + i])
+                    event = 1;
+            }
+        }
+        event = __syncthreads_or(event);
         if (threadIdx.x == 0) {
+            bool stop = false;
+            int reason = 0, flush = 0;
             S.h_did = h;
             S.t = t0 + h;
             S.h_next = h_next;
@@ -385,20 +401,30 @@ __global__ __launch_bounds__(1024) void k_resident(ResSystem *systems, ResArrays
                 S.elapsed += h;
                 S.last_save += h;
                 S.last_nsteps += h;
+                if (event) reason |= SOL_STOP_EVENT;
                 const double actual = 1000.0 * 365.25 * S.millenium + S.t;
                 if (fabs(actual) >= fabs(S.length)) {
                     stop = true;
                 } else {
                     if (fabs(actual + S.h_next) > fabs(S.length)) S.h_next = S.length - actual;
-                    if (fabs(S.last_save) >= fabs(S.output)) S.last_save = 0.0;
+                    if (fabs(S.last_save) >= fabs(S.output)) {
+                        S.last_save = 0.0;
+                        reason |= SOL_STOP_OUTPUT;
+                    }
                     if (fabs(S.last_save + S.h_next) > fabs(S.output)) S.h_next = S.output - S.last_save;
                 }
+                // Counter::succededStep % CheckForSM (:154-159)
+                S.flush_count = (S.flush_count + 1) % 100;
+                flush = S.flush_count == 0;
+                S.reason = reason | (stop ? SOL_STOP_END : 0);
+                reason &= S.stop_mask;
             }
-            sh[1] = stop ? 1.0 : 0.0;
+            sh[1] = stop ? 1.0 : reason ? 2.0 : 0.0;
+            sh[2] = flush ? 1.0 : 0.0;
         }
         __syncthreads();
-        stop = sh[1] != 0.0;
-        if (S.clip && (S.steps % 100) == 0) {
+        const double next = sh[1];
+        if (sh[2] != 0.0) {
             for (int i = threadIdx.x; i < c.n; i += blockDim.x)
                 for (int k = 0; k < 6; ++k) {
                     if (fabs(cat(A.y0, c, k, i)) < 1.0e-50) at(A.y0, c, k, i) = 0.0;
@@ -406,8 +432,8 @@ __global__ __launch_bounds__(1024) void k_resident(ResSystem *systems, ResArrays
                 }
         }
         __syncthreads();
-        if (stop) {
-            if (threadIdx.x == 0) S.status = RES_DONE;
+        if (next != 0.0) {
+            if (threadIdx.x == 0) S.status = next == 1.0 ? RES_DONE : RES_STOPPED;
             return;
         }
     }

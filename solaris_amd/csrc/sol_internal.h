@@ -148,6 +148,8 @@ void launch_errmax_final(hipStream_t st, const double *errblk, int nblk, double 
 void launch_aos_to_soa(hipStream_t st, int n, int np, const double *aos, double *soa);
 void launch_soa_to_aos(hipStream_t st, int n, int np, const double *soa, double *aos);
 void launch_flush_small(hipStream_t st, int n, int np, double *a, double *b);
+void launch_event_check(hipStream_t st, int n, const double *rm3, const int32_t *nni, const double *nnd,
+                        const double *radius, double e3, double h3, double factor, int32_t *flag);
 void launch_integrals(hipStream_t st, int n, int np, const double *y0, const double *mass,
                       double *per_body /* 11 x NP */);
 // adjoint.hip

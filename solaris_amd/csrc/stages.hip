@@ -652,6 +652,27 @@ __global__ void k_flush_small(int n, int np, double *__restrict__ a, double *__r
     }
 }
 
+// Simulator::CheckEvent's thresholds (Solaris/Simulator.cpp:631-644,
+// :688-695) over the side outputs of the last evaluation: one block, a
+// grid-stride loop over the bodies, and one flag (any body crosses a set
+// threshold) written by thread 0.  e3 = 1/ejection^3, h3 = 1/hitCentrum^3
+// and factor are 0 when their check is off.
+__global__ __launch_bounds__(256) void k_event_check(int n, const double *__restrict__ rm3,
+                                                     const int32_t *__restrict__ nni, const double *__restrict__ nnd,
+                                                     const double *__restrict__ radius, double e3, double h3,
+                                                     double factor, int32_t *__restrict__ flag)
+{
+    int ev = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (i >= 1 && e3 > 0.0 && rm3[i] < e3) ev = 1;
+        if (i >= 1 && h3 > 0.0 && rm3[i] > h3) ev = 1;
+        const int j = nni[i];
+        if (factor > 0.0 && j >= 0 && j < n && factor * (radius[i] + radius[j]) > nnd[i]) ev = 1;
+    }
+    ev = __syncthreads_or(ev);
+    if (threadIdx.x == 0) *flag = ev;
+}
+
 // Per-body terms of Calculate::Integrals (Solaris/Calculate.cpp:74-172):
 // rows 0-5 m*y, 6-8 m (r x v), 9 kinetic 0.5 m v^2, 10 sum_j m_i m_j / r_ij.
 __global__ __launch_bounds__(256) void k_integrals(int n, int np, const double *__restrict__ y,
@@ -724,6 +745,12 @@ void launch_soa_to_aos(hipStream_t st, int n, int np, const double *soa, double 
 void launch_flush_small(hipStream_t st, int n, int np, double *a, double *b)
 {
     hipLaunchKernelGGL(k_flush_small, dim3(nblocks(n)), dim3(kBlock), 0, st, n, np, a, b);
+}
+
+void launch_event_check(hipStream_t st, int n, const double *rm3, const int32_t *nni, const double *nnd,
+                        const double *radius, double e3, double h3, double factor, int32_t *flag)
+{
+    hipLaunchKernelGGL(k_event_check, dim3(1), dim3(kBlock), 0, st, n, rm3, nni, nnd, radius, e3, h3, factor, flag);
 }
 
 void launch_integrals(hipStream_t st, int n, int np, const double *y0, const double *mass, double *out)

@@ -57,6 +57,9 @@ _SIGS = {
     "sol_integrate": (C.c_int, [V, C.c_int32, C.c_double, C.POINTER(_abi.TimeLine), C.c_int64,
                                 C.POINTER(_abi.RunStats)]),
     "sol_integrals": (C.c_int, [V, D]),
+    "sol_set_events": (C.c_int, [V, C.POINTER(_abi.EventSettings)]),
+    "sol_integrate_until": (C.c_int, [V, C.c_int32, C.c_double, C.POINTER(_abi.TimeLine), C.c_int64, C.c_int32,
+                                      C.POINTER(C.c_int32), C.POINTER(_abi.RunStats)]),
     "sol_interactions_per_evaluation": (C.c_int, [V, D]),
     "sol_version": (C.c_char_p, []),
     "sol_profile": (C.c_int, [V, C.c_int32, C.POINTER(_abi.KernelProfile)]),
@@ -218,6 +221,18 @@ class Context:
         st = _abi.RunStats()
         r = self.lib.sol_integrate(self.h, int(integrator), float(eps), C.byref(tl), int(max_steps), C.byref(st))
         return r, st
+
+    def set_events(self, ejection=0.0, hit_centrum=0.0, collision_factor=0.0):
+        ev = _abi.EventSettings(float(ejection), float(hit_centrum), float(collision_factor))
+        self._check(self.lib.sol_set_events(self.h, C.byref(ev)))
+
+    def integrate_until(self, integrator, eps, tl, stop_mask, max_steps=0):
+        """Returns (status, reason bits, RunStats)."""
+        st = _abi.RunStats()
+        why = C.c_int32(0)
+        r = self.lib.sol_integrate_until(self.h, int(integrator), float(eps), C.byref(tl), int(max_steps),
+                                         int(stop_mask), C.byref(why), C.byref(st))
+        return r, why.value, st
 
     def integrals(self):
         out = np.zeros(16)

@@ -1,0 +1,161 @@
+"""sol_integrate_until: the caller loop on the device, stopping only where
+the host has work (outputs due, events).
+
+Checked against the same loop driven one accepted step at a time
+(sol_integrate with max_steps = 1, which applies DecisionMaking after the
+step) with the output and event decisions taken on the host, as
+Simulator::DecisionMaking and Simulator::CheckEvent take them
+(Solaris/Simulator.cpp:181-248, :621-735):
+
+* an output is due after a step that reset lastSave to 0;
+* ejection: some body i >= 1 with rm3[i] < 1/ejection^3; hit-centrum:
+  rm3[i] > 1/hitCentrum^3; collision: some body whose nearest neighbour j
+  has factor * (R_i + R_j) > distanceOfNN[i] (the side outputs of the last
+  stage of the accepted step).
+
+Both the workgroup-resident path (small gas-free systems) and the
+multi-kernel path (SOLARIS_AMD_NO_RESIDENT=1) must stop at the same steps,
+with the same time line and bit-identical states.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from solaris_amd import _abi, runtime, scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+RKF78 = S.RUNGE_KUTTA_FEHLBERG78
+
+
+def make_ctx(scn, resident):
+    if not resident:
+        os.environ["SOLARIS_AMD_NO_RESIDENT"] = "1"
+    try:
+        ctx = runtime.Context(scn)
+    finally:
+        os.environ.pop("SOLARIS_AMD_NO_RESIDENT", None)
+    ctx.set_state(scn.y0)
+    return ctx
+
+
+def host_events(ctx, scn, ejection, hit_centrum, factor):
+    rm3, nn, dist, _ = ctx.side_outputs()
+    ev = False
+    if ejection > 0:
+        ev |= bool(np.any(rm3[1:] < 1.0 / ejection ** 3))
+    if hit_centrum > 0:
+        ev |= bool(np.any(rm3[1:] > 1.0 / hit_centrum ** 3))
+    if factor > 0:
+        j = nn
+        ok = j >= 0
+        ev |= bool(np.any(factor * (scn.radius[ok] + scn.radius[j[ok]]) > dist[ok]))
+    return ev
+
+
+def step_loop(ctx, scn, tl, max_steps, ejection=0.0, hit_centrum=0.0, factor=0.0, want_outputs=True):
+    """One accepted step per call; returns the list of (step, time, reason)
+    stops the device loop should produce."""
+    stops = []
+    for k in range(1, max_steps + 1):
+        r, st = ctx.integrate(RKF78, scn.epsilon, tl, max_steps=1)
+        assert r == 0, ctx.last_error()
+        why = 0
+        if (ejection or hit_centrum or factor) and host_events(ctx, scn, ejection, hit_centrum, factor):
+            why |= _abi.STOP_EVENT
+        if want_outputs and tl.lastSave == 0.0:
+            why |= _abi.STOP_OUTPUT
+        actual = 1000.0 * 365.25 * tl.millenium + tl.time
+        if abs(actual) >= abs(tl.length):
+            why |= _abi.STOP_END
+        if why:
+            stops.append((k, tl.time, why, ctx.get_array(runtime.Y0)))
+            if why & (_abi.STOP_EVENT | _abi.STOP_END):
+                break
+    return stops
+
+
+def until_loop(ctx, scn, tl, mask, max_total):
+    stops, total = [], 0
+    while total < max_total:
+        r, why, st = ctx.integrate_until(RKF78, scn.epsilon, tl, mask, max_steps=max_total - total)
+        assert r == 0, ctx.last_error()
+        total += st.accepted_steps
+        if why & ~_abi.STOP_STEPS:
+            stops.append((total, tl.time, why & ~_abi.STOP_STEPS, ctx.get_array(runtime.Y0)))
+        if why & (_abi.STOP_EVENT | _abi.STOP_END) or why & _abi.STOP_STEPS:
+            break
+    return stops
+
+
+def compare(a, b):
+    assert len(a) == len(b), (len(a), len(b))
+    for (ka, ta, wa, ya), (kb, tb, wb, yb) in zip(a, b):
+        assert (ka, wa) == (kb, wb), ((ka, wa), (kb, wb))
+        assert ta == tb
+        assert np.array_equal(ya, yb)
+
+
+@pytest.mark.parametrize("resident", [True, False])
+def test_stops_at_outputs(resident):
+    scn = S.make_c1(n_tp=64)
+    out = 20.0 * 365.25
+    a_ctx, b_ctx = make_ctx(scn, resident), make_ctx(scn, resident)
+    ta = _abi.timeline(0.0, scn.h_next, 100.0 * 365.25, out)
+    tb = _abi.timeline(0.0, scn.h_next, 100.0 * 365.25, out)
+    print(f"[outputs resident={resident}]", flush=True)
+    ref = step_loop(b_ctx, scn, tb, 100000)
+    got = until_loop(a_ctx, scn, ta, _abi.STOP_OUTPUT, 100000)
+    print(f"  {len(got)} stops, last at step {got[-1][0]} t = {got[-1][1]:.3f} d", flush=True)
+    compare(got, ref)
+    assert [s[2] for s in got][-1] & _abi.STOP_END
+    assert ta.flush_count == got[-1][0] % 100 == tb.flush_count
+    for f in ("time", "hNext", "hDid", "elapsedTime", "lastSave", "lastNSteps"):
+        assert getattr(ta, f) == getattr(tb, f), f
+
+
+def event_system(kind):
+    """C1 with 64 test particles (3 <= a <= 12 AU, e <= 0.1: r from 3.1 to
+    12.8 AU), plus one particle that crosses the event radius some years in:
+    for ejection (14 AU) one leaving perihelion at 4 AU on an orbit out to
+    16 AU; for hit-centrum (2.5 AU) one leaving aphelion at 8 AU on an orbit
+    in to 2 AU.  For collisions the giants get radii of 0.3 AU, so that a
+    particle passing one of them meets factor * (R_i + R_j) > d_NN."""
+    scn = S.make_c1(n_tp=64)
+    scn.y0 = scn.y0.copy()
+    scn.radius = scn.radius.copy()
+    mu = np.array([S.GAUSS2 * scn.mass[0]])
+    one = lambda x: np.array([x])
+    if kind == "ejection":
+        scn.y0[-1] = S.elements_to_cartesian(mu, one(10.0), one(0.6), one(0.01), one(0.3), one(1.1), one(0.0))[0]
+        return scn, (14.0, 0.0, 0.0)
+    if kind == "hit_centrum":
+        scn.y0[-1] = S.elements_to_cartesian(mu, one(5.0), one(0.6), one(0.01), one(2.0), one(0.4), one(np.pi))[0]
+        return scn, (0.0, 2.5, 0.0)
+    scn.radius[1:3] = 0.3
+    return scn, (0.0, 0.0, 1.0)
+
+
+@pytest.mark.parametrize("resident", [True, False])
+@pytest.mark.parametrize("kind", ["ejection", "hit_centrum", "collision"])
+def test_stops_at_events(resident, kind):
+    scn, ev = event_system(kind)
+    a_ctx, b_ctx = make_ctx(scn, resident), make_ctx(scn, resident)
+    a_ctx.set_events(*ev)
+    ta = _abi.timeline(0.0, scn.h_next, 2000.0 * 365.25, 50.0 * 365.25)
+    tb = _abi.timeline(0.0, scn.h_next, 2000.0 * 365.25, 50.0 * 365.25)
+    print(f"[events {kind} resident={resident}]", flush=True)
+    ref = step_loop(b_ctx, scn, tb, 200000, *ev, want_outputs=False)
+    got = until_loop(a_ctx, scn, ta, _abi.STOP_EVENT, 200000)
+    assert ref and ref[-1][2] & _abi.STOP_EVENT, "the host loop found no event: the test system needs adjusting"
+    print(f"  event at step {got[-1][0]} t = {got[-1][1] / 365.25:.3f} yr", flush=True)
+    assert ref[-1][0] > 20, "the event must come after some steps, not at the start"
+    compare(got, ref)
+
+
+def test_set_events_rejects_negative():
+    scn = S.make_c1(n_tp=8)
+    ctx = runtime.Context(scn)
+    with pytest.raises(runtime.SolarisError):
+        ctx.set_events(-1.0, 0.0, 0.0)
