@@ -156,6 +156,87 @@ def adjoint_timing():
                 forward_interactions_per_s=4 * 2000 * ipe / (t1 - t0))
 
 
+def reference_steps_per_s(scn, seconds: float, max_steps: int):
+    """The compiled reference's RungeKuttaFehlberg78::Driver (oracle/_ref),
+    single thread, from the scenario's initial state: accepted steps per
+    CPU second over a bounded sample."""
+    from oracle import cpu_ref
+    from solaris_amd import _abi, scenarios as S
+    if not cpu_ref.available("ref"):
+        return None
+    ref = cpu_ref.Reference(scn)
+    ref.set_state(scn.y0)
+    tl = _abi.timeline(0.0, scn.h_next, 1.0e12)
+    n = 0
+    t0 = time.process_time()
+    while n < max_steps and time.process_time() - t0 < seconds:
+        if ref.driver(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl)[0] != 0:
+            break
+        n += 1
+    dt = time.process_time() - t0
+    ref.close()
+    return dict(steps_per_s=n / dt, cores=1, kind="reference",
+                sample=f"{n} accepted RKF78 steps from the initial state, {dt:.1f} s CPU")
+
+
+def c1_caller_loop():
+    """C1: star, Jupiter, Saturn and 512 test particles, RKF78 1e-10, 1,000 yr
+    with an output every 10 yr: sol_integrate_until stops at each output
+    (where a caller saves phases and integrals) and at the end."""
+    from solaris_amd import _abi, runtime, scenarios as S
+    scn = S.make_c1()
+    ctx = runtime.Context(scn)
+    ctx.set_state(scn.y0)
+    tl = _abi.timeline(0.0, scn.h_next, scn.length, 10.0 * 365.25)
+    stops = steps = 0
+    t0 = time.perf_counter()
+    while True:
+        r, why, st = ctx.integrate_until(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl, _abi.STOP_OUTPUT)
+        if r:
+            raise RuntimeError(ctx.last_error())
+        steps += st.accepted_steps
+        stops += 1
+        if why & _abi.STOP_END:
+            break
+    wall = time.perf_counter() - t0
+    ctx.close()
+    out = dict(workload="C1: star + Jupiter + Saturn + 512 test particles, RKF78 1e-10, 1000 yr, "
+                        "output every 10 yr (sol_integrate_until)",
+               wall_s=wall, accepted_steps=steps, stops=stops, steps_per_s=steps / wall)
+    cpu = reference_steps_per_s(scn, 3.0, steps)
+    if cpu:
+        cpu["wall_s_extrapolated"] = steps / cpu["steps_per_s"]
+        out["cpu_reference"] = cpu
+        out["speedup_gpu_over_cpu"] = out["steps_per_s"] / cpu["steps_per_s"]
+    return out
+
+
+def c2_steps(nsteps: int = 1000):
+    """C2: star, 4 migrating protoplanets and 16,384 planetesimals with gas
+    drag and Type I migration, RKF78 1e-10, nsteps accepted steps."""
+    from solaris_amd import _abi, runtime, scenarios as S
+    scn = S.make_c2()
+    ctx = runtime.Context(scn)
+    ctx.set_state(scn.y0)
+    tl = _abi.timeline(0.0, scn.h_next, 1.0e12)
+    r, _ = ctx.integrate(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl, max_steps=5)   # warm-up
+    t0 = time.perf_counter()
+    r, st = ctx.integrate(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl, max_steps=nsteps)
+    wall = time.perf_counter() - t0
+    if r:
+        raise RuntimeError(ctx.last_error())
+    ctx.close()
+    out = dict(workload=f"C2: star + 4 migrating protoplanets + 16,384 planetesimals with gas drag, RKF78 1e-10, "
+                        f"{nsteps} steps", wall_s=wall, accepted_steps=int(st.accepted_steps),
+               attempts=int(st.attempts), steps_per_s=st.accepted_steps / wall,
+               interactions_per_s=st.interactions / wall)
+    cpu = reference_steps_per_s(scn, 5.0, nsteps)
+    if cpu:
+        out["cpu_reference"] = cpu
+        out["speedup_gpu_over_cpu"] = out["steps_per_s"] / cpu["steps_per_s"]
+    return out
+
+
 def traffic_probe(n: int):
     """Launch sequence profiled by the two `rocprofv3 --pmc` passes
     (FETCH_SIZE, then WRITE_SIZE) that give the pair kernel's HBM traffic:
@@ -208,7 +289,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the ensemble, adjoint and all-core CPU figures")
+                    help="skip the ensemble, adjoint, C1, C2 and all-core CPU figures")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the launch sequence for the FETCH_SIZE / WRITE_SIZE counter passes")
     args = ap.parse_args()
@@ -294,7 +375,8 @@ def main():
         except runtime.SolarisError as e:
             line["mixed"] = {"value": None, "note": str(e)}
     if not args.no_extras:
-        for key, fn in (("ensemble", lambda: ensemble_throughput(4096, 10.0)), ("adjoint", adjoint_timing)):
+        for key, fn in (("ensemble", lambda: ensemble_throughput(4096, 10.0)), ("adjoint", adjoint_timing),
+                        ("c1", c1_caller_loop), ("c2", c2_steps)):
             try:
                 line[key] = fn()
             except Exception as e:      # reported, not fatal: the headline figure stands alone
