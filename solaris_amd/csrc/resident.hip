@@ -42,12 +42,17 @@ __device__ __forceinline__ double wmax(double v)
     return v;
 }
 
-// Block-wide sum of v[0..5] (fixed order); result in out[0..5] for every thread.
-__device__ void block_sum6_all(double v[6], double *red /* [16][6] */, double *out /* [6] */)
+// Block-wide sum of v[0..5] (fixed order); result in out[0..5] for every
+// thread.  A wave whose v are all +0.0 by construction (wave_zero, a
+// wave-uniform flag) skips its shuffle reduction: its partial is +0.0
+// either way.
+__device__ void block_sum6_all(double v[6], double *red /* [16][6] */, double *out /* [6] */,
+                               bool wave_zero = false)
 {
     const int wid = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
     const int nw = blockDim.x / kWave;
-    for (int k = 0; k < 6; ++k) v[k] = wsum(v[k]);
+    if (!wave_zero)
+        for (int k = 0; k < 6; ++k) v[k] = wsum(v[k]);
     if (lane == 0)
         for (int k = 0; k < 6; ++k) red[wid * 6 + k] = v[k];
     __syncthreads();
@@ -108,12 +113,14 @@ template <typename Get>
 __device__ void stage_positions(const Ctx &c, Get get)
 {
     const ResSystem &S = *c.S;
+    // only the bodies below the larger source end enter the indirect sums
+    const int end_max = S.end_a > S.end_b ? S.end_a : S.end_b;
     double v[6] = {0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < c.n; i += blockDim.x) {
         const double x = get(i, 0), y = get(i, 1), z = get(i, 2);
         const double gm = kGauss2 * c.A->mass[c.base + i];
         c.sJ[i] = make_double4(x, y, z, gm);
-        if (!S.barycentric && i >= 1) {
+        if (!S.barycentric && i >= 1 && i < end_max) {
             const double r2 = x * x + y * y + z * z;
             const double r = sqrt(r2);
             const double rm3 = 1.0 / (r2 * r);
@@ -122,7 +129,10 @@ __device__ void stage_positions(const Ctx &c, Get get)
             if (i < S.end_b) { v[3] += cx; v[4] += cy; v[5] += cz; }
         }
     }
-    block_sum6_all(v, c.red, c.ind);   // includes the barrier that publishes sJ
+    // a wave whose first body is past the source end holds no source
+    // (its bodies are wave_first + q * blockDim, all >= end_max)
+    const bool wave_zero = S.barycentric || (int)(threadIdx.x & ~(kWave - 1)) >= end_max;
+    block_sum6_all(v, c.red, c.ind, wave_zero);   // includes the barrier that publishes sJ
 }
 
 // Acceleration of body i from the LDS positions (GravityAC / GravityBC
@@ -205,31 +215,45 @@ __device__ void evaluate(const Ctx &c, int l, bool last, Vel vel)
     __syncthreads();   // sJ may be overwritten by the next stage
 }
 
-// Next-stage state of body i, component comp (first-order tableaux).
+// Next-stage state of body i, component comp (first-order tableaux).  The
+// stage derivatives are read first, all together, so that their latencies
+// overlap; the sum then takes the nonzero coefficients in the tableau's
+// order, as the reference's hand-written stage sums do.
 __device__ double combine1(const Ctx &c, const double *D, int sn, int i, int comp, double h)
 {
+    double f[13];
+#pragma unroll
+    for (int l = 0; l < 13; ++l) f[l] = l < sn ? F(c, l, comp, i) : 0.0;
+    const double y0 = cat(c.A->y0, c, comp, i);
     double acc = 0.0;
     bool first = true;
-    for (int l = 0; l < sn; ++l) {
+#pragma unroll
+    for (int l = 0; l < 13; ++l) {
+        if (l >= sn) break;
         const double d = D[l];
         if (d == 0.0) continue;
-        const double v = d * F(c, l, comp, i);
+        const double v = d * f[l];
         acc = first ? v : acc + v;
         first = false;
     }
-    return cat(c.A->y0, c, comp, i) + h * acc;
+    return y0 + h * acc;
 }
 
 // Dormand-Prince RKN stage state (component j < 3 position, j + 3 velocity).
 __device__ void combine_dp(const Ctx &c, int sn, int i, int j, double h, double &pos, double &vel)
 {
     const double *A = c.coef + kDpA + sn * 8;
-    double var = A[0] * F(c, 0, j + 3, i);
-    for (int l = 1; l < sn; ++l) {
-        if (A[l] == 0.0) continue;
-        var += A[l] * F(c, l, j + 3, i);
-    }
+    double f[9];
+#pragma unroll
+    for (int l = 0; l < 9; ++l) f[l] = l < sn ? F(c, l, j + 3, i) : 0.0;
     const double p0 = cat(c.A->y0, c, j, i), v0 = cat(c.A->y0, c, j + 3, i);
+    double var = A[0] * f[0];
+#pragma unroll
+    for (int l = 1; l < 9; ++l) {
+        if (l >= sn) break;
+        if (A[l] == 0.0) continue;
+        var += A[l] * f[l];
+    }
     pos = p0 + c.coef[kDpC + sn] * h * v0 + h * h * (var);
     vel = v0 + h * (var);
 }
@@ -445,7 +469,7 @@ int resident_block_size(int n)
 {
     int b = ((n + kWave - 1) / kWave) * kWave;
     if (b < kWave) b = kWave;
-    if (b > 512) b = 512;
+    if (b > 1024) b = 1024;
     return b;
 }
 
