@@ -371,7 +371,9 @@ def main():
                              "pair_kernel": "k_pairs_sym_mixed<true>" if m["pair_sym_launches"] else
                                             "k_pairs_mixed<false, false, true>",
                              "pair_ms_per_launch": (m["pair_sym_ms"] / m["pair_sym_launches"]) if m["pair_sym_launches"]
-                                                   else m["pair_ms"] / max(m["pair_launches"], 1)}
+                                                   else m["pair_ms"] / max(m["pair_launches"], 1),
+                             "nn_search_ms_per_launch": m["pair_nn_ms"] / max(m["pair_nn_launches"], 1),
+                             "nn_search_launches": m["pair_nn_launches"]}
         except runtime.SolarisError as e:
             line["mixed"] = {"value": None, "note": str(e)}
     if not args.no_extras:

@@ -344,13 +344,26 @@ int upload_scalars(sol_context *c, double h, double t)
 }
 
 // The pair sums of one evaluation from the current J (and J32): the
-// symmetric sum of the mutual block plus the rest when possible (no
-// nearest-neighbour search, sym_ok), else the one-directional sum.  Sets
-// the epilogue's plan fields.
-void pair_sums(sol_context *c, bool nn, bool sym_ok, EpiArgs &a)
+// symmetric sum of the mutual block plus the one-directional sum of the
+// rest when the context has a symmetric plan (astrocentric, block of at
+// least 2,048 bodies), else the one-directional sum of everything.
+//
+// With nn (the last stage of an attempt, sol_compute with side outputs),
+// FP64 keeps the one-directional kernel, whose nearest-neighbour tracking
+// costs 0.5 ms on top of its 2.5 ms at 65,537 bodies, less than the
+// symmetric sum plus a separate search (1.6 + 1.6 ms).  Mixed precision
+// uses the symmetric sum plus the separate FP32 search: every stage then
+// sums its FP32 terms in the same kernel, so their rounding cancels in the
+// error estimate's F10 - F12 (RKF78) and F7 - F8 (Dormand-Prince), both
+// evaluated at the end of the step; with the one-directional kernel at
+// the last stage only, the estimate carried two unrelated FP32 roundings
+// and, on a velocity component near zero (small yscale), rejected steps
+// that FP64 accepts.  Sets the epilogue's plan fields.
+void pair_sums(sol_context *c, bool nn, EpiArgs &a)
 {
     const double eps2 = c->softening * c->softening;
-    if (c->sym.M > 0 && !nn && sym_ok) {
+    a.nn_sep = 0;
+    if (c->sym.M > 0 && (!nn || mixed_on(c))) {
         {
             ProfScope ps(c, 3, c->sym_inter_per_launch);
             if (mixed_on(c))
@@ -362,6 +375,12 @@ void pair_sums(sol_context *c, bool nn, bool sym_ok, EpiArgs &a)
             ProfScope ps(c, 0, c->rest_inter_per_launch);
             launch_pairs(c->stream, c->plan_rest, c->J, c->part, c->nnr2, c->nnidx, false, false, mixed_on(c), eps2,
                          c->J32);
+        }
+        if (nn) {
+            ProfScope ps(c, 2, c->pair_inter_per_launch);
+            launch_nn_only(c->stream, c->plan, c->J, c->J32, c->nnr2, c->nnidx, c->barycentric != 0, mixed_on(c));
+            a.nnplan = c->plan;
+            a.nn_sep = 1;
         }
         a.plan = c->plan_rest;
         a.sym = c->sym;
@@ -401,7 +420,7 @@ int evaluate_at(sol_context *c, const double *st, double t, double h, double *ou
                              c->stop, c->migt, c->migc, c->gasdev, c->barycentric, t, do1, do2);
     }
     EpiArgs a = base_args(c);
-    pair_sums(c, last, true, a);
+    pair_sums(c, last, a);
     a.kind = kind;
     a.integ = integ;
     a.stage = 0;
@@ -436,19 +455,6 @@ int integ_kind(int32_t integrator)
 
 int n_stages(int ik) { return ik == INTEG_RKF78 ? 12 : ik == INTEG_RK4 ? 3 : 8; }
 
-// The stage whose derivative enters the error estimate against the last
-// stage's: F10 - F12 in RungeKuttaFehlberg78 (Solaris/RungeKuttaFehlberg78.cpp:242),
-// F7 - F8 in DormandPrince (Solaris/DormandPrince.cpp:477); -1 for RungeKutta4.
-// Both stages are evaluated at the end point of the step, so in FP64 the
-// difference is pure truncation error.  In mixed precision the FP32 rounding
-// of the pair sums cancels in that difference only when both stages use the
-// same kernel (same FP32 inputs to within rounding, same summation order):
-// with the symmetric kernel at one and the one-directional kernel (the
-// nearest-neighbour search) at the other, the error estimate carries the
-// difference of two FP32 roundings, which on a velocity component near zero
-// (small yscale) exceeds the tolerance and rejects good steps.  F0 - F11
-// needs nothing: both stages use the symmetric kernel.
-int last_partner_stage(int ik) { return ik == INTEG_RKF78 ? 10 : ik == INTEG_DP ? 7 : -1; }
 
 // Stage evaluations 1..S of one Step from Y0 and ACCEL at (t, h) into Y,
 // ERROR and the error maximum (left in c->h_scal[2] after the sync when
@@ -468,7 +474,7 @@ int run_stages(sol_context *c, int ik, double t, double h, bool read_err, double
     for (int s = 1; s <= S; ++s) {
         const bool last = (s == S);
         EpiArgs e = a;
-        pair_sums(c, last, !(mixed_on(c) && s == last_partner_stage(ik)), e);
+        pair_sums(c, last, e);
         e.stage = s;
         e.kind = last ? EPI_FINAL : EPI_MID;
         e.last = last ? 1 : 0;

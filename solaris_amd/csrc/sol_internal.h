@@ -98,6 +98,8 @@ struct EpiArgs {
     SymPlan sym;              // symmetric pair sum of the mutual block (used when use_sym)
     int32_t use_sym, pad3;
     const double *apart, *bpart;   // its row-side and column-side records
+    PairPlan nnplan;          // nearest-neighbour slots of a separate search (used when nn_sep)
+    int32_t nn_sep, pad4;
 };
 
 // Arguments of one force vector-Jacobian product (adjoint.hip).  Y is the
@@ -128,6 +130,11 @@ void fill_coefficients(std::vector<double> &c);
 void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
                   double *nnr2, int32_t *nnidx, bool nn, bool barycentric, bool mixed,
                   double eps2, const float4 *J32);
+// Nearest-neighbour search alone over `plan` (slots of nnr2 / nnidx as the
+// NN variant of launch_pairs writes them); fp32: single-precision search
+// from J32.
+void launch_nn_only(hipStream_t st, const PairPlan &plan, const double4 *J, const float4 *J32, double *nnr2,
+                    int32_t *nnidx, bool bc, bool fp32);
 // Symmetric pair sum over the mutual block (sym_plan.h): row-side records
 // in apart, column-side records in bpart.
 void launch_pairs_sym(hipStream_t st, const SymPlan &sp, const double4 *J, double *apart, double *bpart,
