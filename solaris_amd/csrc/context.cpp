@@ -16,67 +16,22 @@
 #include <vector>
 
 #include "context.h"
+#include "tableaux.h"
 
 using namespace sol;
 
-// Fehlberg 7(8) coefficients D_{s,l} (NASA TR R-381; as set in
-// RungeKuttaFehlberg78.cpp:41-56), RK4 tableau (RungeKutta4.cpp:60-70) and
-// Dormand-Prince RKN 7(6) (DormandPrince.cpp:37-123), packed at the offsets
-// stages.hip and resident.hip read.
+// The tables of tableaux.h packed at the offsets stages.hip reads.
 void sol::fill_coefficients(std::vector<double> &c)
 {
     c.assign(275, 0.0);
-    const double D[13][12] = {
-        {0},
-        {2.0 / 27.0},
-        {1.0 / 36.0, 1.0 / 12.0},
-        {1.0 / 24.0, 0, 1.0 / 8.0},
-        {5.0 / 12.0, 0, -25.0 / 16.0, 25.0 / 16.0},
-        {1.0 / 20.0, 0, 0, 1.0 / 4.0, 1.0 / 5.0},
-        {-25.0 / 108.0, 0, 0, 125.0 / 108.0, -65.0 / 27.0, 125.0 / 54.0},
-        {31.0 / 300.0, 0, 0, 0, 61.0 / 225.0, -2.0 / 9.0, 13.0 / 900.0},
-        {2.0, 0, 0, -53.0 / 6.0, 704.0 / 45.0, -107.0 / 9.0, 67.0 / 90.0, 3.0},
-        {-91.0 / 108.0, 0, 0, 23.0 / 108.0, -976.0 / 135.0, 311.0 / 54.0, -19.0 / 60.0, 17.0 / 6.0, -1.0 / 12.0},
-        {2383.0 / 4100.0, 0, 0, -341.0 / 164.0, 4496.0 / 1025.0, -301.0 / 82.0, 2133.0 / 4100.0, 45.0 / 82.0,
-         45.0 / 164.0, 18.0 / 41.0},
-        {3.0 / 205.0, 0, 0, 0, 0, -6.0 / 41.0, -3.0 / 205.0, -3.0 / 41.0, 3.0 / 41.0, 6.0 / 41.0},
-        {-1777.0 / 4100.0, 0, 0, -341.0 / 164.0, 4496.0 / 1025.0, -289.0 / 82.0, 2193.0 / 4100.0, 51.0 / 82.0,
-         33.0 / 164.0, 12.0 / 41.0, 0, 1.0},
-    };
     for (int s = 0; s < 13; ++s)
-        for (int l = 0; l < 12; ++l) c[0 + s * 12 + l] = D[s][l];
-    // RK4: a21 = a32 = 1/2, a43 = 1; b = (1/6, 1/3, 1/3, 1/6); c = (0, 1/2, 1/2, 1)
-    c[156 + 1 * 3 + 0] = 0.5;
-    c[156 + 2 * 3 + 1] = 0.5;
-    c[156 + 3 * 3 + 2] = 1.0;
-    c[168 + 0] = 1.0 / 6.0; c[168 + 1] = 1.0 / 3.0; c[168 + 2] = 1.0 / 3.0; c[168 + 3] = 1.0 / 6.0;
-    c[172 + 0] = 0.0; c[172 + 1] = 0.5; c[172 + 2] = 0.5; c[172 + 3] = 1.0;
-    // Dormand-Prince RKN 7(6)
-    const double sq = std::sqrt(21.0);
-    double a[9][8] = {};
-    a[1][0] = 1.0 / 200.0;
-    a[2][0] = 1.0 / 150.0; a[2][1] = 1.0 / 75.0;
-    a[3][0] = 171.0 / 8192.0; a[3][1] = 45.0 / 4096.0; a[3][2] = 315.0 / 8192.0;
-    a[4][0] = 5.0 / 288.0; a[4][1] = 25.0 / 528.0; a[4][2] = 25.0 / 672.0; a[4][3] = 16.0 / 693.0;
-    a[5][0] = (1003.0 - 205.0 * sq) / 12348.0; a[5][1] = -25.0 * (751.0 - 173.0 * sq) / 90552.0;
-    a[5][2] = 25.0 * (624.0 - 137.0 * sq) / 43218.0; a[5][3] = -128.0 * (361.0 - 79.0 * sq) / 237699.0;
-    a[5][4] = (3411.0 - 745.0 * sq) / 24696.0;
-    a[6][0] = (793.0 + 187.0 * sq) / 12348.0; a[6][1] = -25.0 * (331.0 + 113.0 * sq) / 90552.0;
-    a[6][2] = 25.0 * (1044.0 + 247.0 * sq) / 43218.0; a[6][3] = -128.0 * (14885.0 + 3779.0 * sq) / 9745659.0;
-    a[6][4] = (3327.0 + 797.0 * sq) / 24696.0; a[6][5] = -(581.0 + 127.0 * sq) / 1722.0;
-    a[7][0] = -(157.0 - 3.0 * sq) / 378.0; a[7][1] = 25.0 * (143.0 - 10.0 * sq) / 2772.0;
-    a[7][2] = -25.0 * (876.0 + 55.0 * sq) / 3969.0; a[7][3] = 1280.0 * (913.0 + 18.0 * sq) / 596673.0;
-    a[7][4] = -(1353.0 + 26.0 * sq) / 2268.0; a[7][5] = 7.0 * (1777.0 + 377.0 * sq) / 4428.0;
-    a[7][6] = 7.0 * (5.0 - sq) / 36.0;
-    a[8][0] = 1.0 / 20.0; a[8][4] = 8.0 / 45.0; a[8][5] = 7.0 * (7.0 + sq) / 360.0; a[8][6] = 7.0 * (7.0 - sq) / 360.0;
+        for (int l = 0; l < 12; ++l) c[0 + s * 12 + l] = tab::kRkfD[s][l];
+    for (int s = 0; s < 4; ++s)
+        for (int l = 0; l < 3; ++l) c[156 + s * 3 + l] = tab::kRk4A[s][l];
+    for (int k = 0; k < 4; ++k) { c[168 + k] = tab::kRk4B[k]; c[172 + k] = tab::kRk4C[k]; }
     for (int k = 0; k < 9; ++k)
-        for (int l = 0; l < 8; ++l) c[176 + k * 8 + l] = a[k][l];
-    const double b[9] = {1.0 / 20.0, 0, 0, 0, 8.0 / 45.0, 7.0 * (7.0 + sq) / 360.0, 7.0 * (7.0 - sq) / 360.0,
-                         -1.0 / 20.0, 1.0 / 20.0};
-    const double bdh[9] = {1.0 / 20.0, 0, 0, 0, 16.0 / 45.0, 49.0 / 180.0, 49.0 / 180.0, 1.0 / 20.0, 0.0};
-    const double cc[9] = {0.0, 1.0 / 10.0, 1.0 / 5.0, 3.0 / 8.0, 1.0 / 2.0, (7.0 - sq) / 14.0, (7.0 + sq) / 14.0,
-                          1.0, 1.0};
-    for (int k = 0; k < 9; ++k) { c[248 + k] = b[k]; c[257 + k] = bdh[k]; c[266 + k] = cc[k]; }
+        for (int l = 0; l < 8; ++l) c[176 + k * 8 + l] = tab::kDpA[k][l];
+    for (int k = 0; k < 9; ++k) { c[248 + k] = tab::kDpB[k]; c[257 + k] = tab::kDpBdh[k]; c[266 + k] = tab::kDpC[k]; }
 }
 
 namespace {
@@ -544,7 +499,7 @@ int resident_run(sol_context *c, int ik, double eps, sol_timeline *tl, int64_t m
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(20000, (int64_t)(0.2 / per_step)));
     for (;;) {
         const int64_t k = max_steps > 0 ? std::min(chunk, max_steps - S.steps) : chunk;
-        launch_resident(c->stream, ik, c->res_d, 1, A, (int)k, c->coef, block, c->n);
+        launch_resident(c->stream, ik, c->res_d, 1, A, (int)k, block, c->n);
         HIPCHECK(c, hipGetLastError());
         HIPCHECK(c, hipMemcpyAsync(c->res_h, c->res_d, sizeof(ResSystem), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));

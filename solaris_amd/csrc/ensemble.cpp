@@ -42,7 +42,7 @@ struct sol_ensemble {
     // device
     ResSystem *sys_d = nullptr;
     double *y0 = nullptr, *y = nullptr, *acc0 = nullptr, *yscale = nullptr, *errv = nullptr, *fst = nullptr;
-    double *mass = nullptr, *rm3 = nullptr, *nnd = nullptr, *coef = nullptr;
+    double *mass = nullptr, *rm3 = nullptr, *nnd = nullptr;
     int32_t *nni = nullptr;
     // a bucket: slots [first, first + count), largest system nmax; block is
     // the chosen block size, cand the sizes still to time
@@ -141,7 +141,7 @@ int launch_round(sol_ensemble *e, int ik, int steps)
             ECHECK(e, hipEventCreate(&t0));
             ECHECK(e, hipEventCreate(&t1));
             (void)hipEventRecord(t0, e->stream);
-            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, e->coef, block, b.nmax);
+            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, block, b.nmax);
             ECHECK(e, hipGetLastError());
             (void)hipEventRecord(t1, e->stream);
             (void)hipEventSynchronize(t1);
@@ -151,7 +151,7 @@ int launch_round(sol_ensemble *e, int ik, int steps)
             (void)hipEventDestroy(t1);
             if (b.best_ms <= 0.0 || ms < b.best_ms) { b.best_ms = ms; b.block = block; }
         } else {
-            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, e->coef, b.block, b.nmax);
+            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, b.block, b.nmax);
             ECHECK(e, hipGetLastError());
         }
     }
@@ -206,8 +206,9 @@ int sol_ensemble_create(sol_ensemble **out, int32_t nsys, const sol_nbodies *nbs
         int t = s, nmax = 0;
         while (t < nsys && (nbs[order[t]].total - 1) / 64 == cls) { nmax = std::max(nmax, nbs[order[t]].total); ++t; }
         sol_ensemble::Bucket bk{s, t - s, nmax, resident_block_size(nmax), 0.0, {}};
+        // at most 4 bodies per thread (the resident kernel's widest form)
         for (int c : {64, 128, 256, 512})
-            if (c <= resident_block_size(nmax)) bk.cand.push_back(c);
+            if (c <= resident_block_size(nmax) && 4 * c >= nmax) bk.cand.push_back(c);
         e->buckets.push_back(bk);
         s = t;
     }
@@ -226,17 +227,10 @@ int sol_ensemble_create(sol_ensemble **out, int32_t nsys, const sol_nbodies *nbs
         (r = ealloc(e, &e->acc0, 6 * tot)) || (r = ealloc(e, &e->yscale, 6 * tot)) ||
         (r = ealloc(e, &e->errv, 6 * tot)) || (r = ealloc(e, &e->fst, 12 * 6 * tot)) ||
         (r = ealloc(e, &e->mass, tot)) || (r = ealloc(e, &e->rm3, tot)) || (r = ealloc(e, &e->nnd, tot)) ||
-        (r = ealloc(e, &e->nni, tot)) || (r = ealloc(e, &e->coef, 275))) {
+        (r = ealloc(e, &e->nni, tot))) {
         g_ens_create_error = e->err;
         sol_ensemble_destroy(e);
         return r;
-    }
-    std::vector<double> coef;
-    fill_coefficients(coef);
-    if (hipMemcpy(e->coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-        g_ens_create_error = "coefficient upload failed";
-        sol_ensemble_destroy(e);
-        return SOL_ERR_DEVICE;
     }
     e->y0_h.assign(6 * tot, 0.0);
     e->mass_h.assign(tot, 0.0);
@@ -262,7 +256,7 @@ int sol_ensemble_destroy(sol_ensemble *e)
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->graph) (void)hipGraphExecDestroy(e->graph);
-    void *ptrs[] = {e->sys_d, e->y0, e->y, e->acc0, e->yscale, e->errv, e->fst, e->mass, e->rm3, e->nnd, e->nni, e->coef};
+    void *ptrs[] = {e->sys_d, e->y0, e->y, e->acc0, e->yscale, e->errv, e->fst, e->mass, e->rm3, e->nnd, e->nni};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (e->stream) (void)hipStreamDestroy(e->stream);

@@ -41,6 +41,6 @@ struct ResArrays {
 int resident_block_size(int n);
 size_t resident_lds_bytes(int nmax);
 void launch_resident(hipStream_t st, int integ, ResSystem *systems, int nsys, const ResArrays &A, int max_steps,
-                     const double *coef, int block, int nmax);
+                     int block, int nmax);
 
 }  // namespace sol
