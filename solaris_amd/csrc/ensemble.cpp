@@ -179,6 +179,23 @@ int sol_ensemble_create(sol_ensemble **out, int32_t nsys, const sol_nbodies *nbs
         }
     }
     if (hipSetDevice(device) != hipSuccess) { g_ens_create_error = "hipSetDevice failed"; return SOL_ERR_DEVICE; }
+    {
+        // the resident kernel's LDS (64 B per body of the largest system)
+        // must fit a workgroup's share, with 2 KB left for its fixed arrays
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+            g_ens_create_error = "hipGetDeviceProperties failed";
+            return SOL_ERR_DEVICE;
+        }
+        int nmax = 0;
+        for (int k = 0; k < nsys; ++k) nmax = std::max(nmax, nbs[k].total);
+        if (resident_lds_bytes(nmax) + 2048 > prop.sharedMemPerBlock) {
+            g_ens_create_error = "systems of " + std::to_string(nmax) + " bodies need " +
+                                 std::to_string(resident_lds_bytes(nmax)) + " B of LDS per workgroup; the device has " +
+                                 std::to_string(prop.sharedMemPerBlock);
+            return SOL_ERR_ARGUMENT;
+        }
+    }
     sol_ensemble *e = new sol_ensemble();
     e->device = device;
     e->nsys = nsys;
@@ -206,9 +223,13 @@ int sol_ensemble_create(sol_ensemble **out, int32_t nsys, const sol_nbodies *nbs
         int t = s, nmax = 0;
         while (t < nsys && (nbs[order[t]].total - 1) / 64 == cls) { nmax = std::max(nmax, nbs[order[t]].total); ++t; }
         sol_ensemble::Bucket bk{s, t - s, nmax, resident_block_size(nmax), 0.0, {}};
-        // at most 4 bodies per thread (the resident kernel's widest form)
+        // the exact fit (one body per thread up to 640 bodies) and the powers
+        // of two below it, at most 4 bodies per thread (the resident
+        // kernel's widest form)
+        const int fit = resident_block_size(nmax);
         for (int c : {64, 128, 256, 512})
-            if (c <= resident_block_size(nmax) && 4 * c >= nmax) bk.cand.push_back(c);
+            if (c < fit && 4 * c >= nmax) bk.cand.push_back(c);
+        bk.cand.push_back(fit);
         e->buckets.push_back(bk);
         s = t;
     }

@@ -9,7 +9,7 @@ namespace sol {
 // RES_STOPPED: the step met a stop reason of stop_mask (SOL_STOP_OUTPUT,
 // SOL_STOP_EVENT), recorded in reason; the host resumes with RES_RUNNING.
 enum { RES_RUNNING = 0, RES_DONE = 1, RES_UNDERFLOW = 2, RES_MAXITER = 3, RES_STOPPED = 4 };
-constexpr int kResidentMaxBodies = 2000;   // LDS: 32 B per body
+constexpr int kResidentMaxBodies = 2000;   // LDS: 64 B per body
 
 // One system: its bodies are [base, base + n) of the concatenated arrays.
 // Time-line fields follow TimeLine (Solaris/TimeLine.h); clip switches on

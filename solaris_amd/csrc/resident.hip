@@ -72,7 +72,7 @@ struct Ctx {
     bool bc;
     int64_t tot;
     double4 *sJ;        // LDS: positions + k^2 m of the stage
-    double *red;        // LDS scratch
+    double4 *sT;        // LDS: r_j rm3_j of the indirect-term sources
 };
 
 __device__ __forceinline__ double &at(double *p, const Ctx &c, int comp, int i) { return p[comp * c.tot + c.base + i]; }
@@ -124,132 +124,130 @@ constexpr double dcoef(int s, int l) { return INTEG == INTEG_RKF78 ? tab::kRkfD[
 // Own bodies of this thread: i = threadIdx.x + q * blockDim.x, q < BPT.
 __device__ __forceinline__ int own(int q) { return threadIdx.x + q * blockDim.x; }
 
-// Publishes the stage positions of the own bodies to LDS and forms the
-// indirect-term sums of both source ranges (AC) in ind (sum over the first
-// source range in ind[0..2], over the second in ind[3..5]); the barrier
-// inside makes sJ visible to every sink.
+// Publishes the stage positions of the own bodies to LDS: sJ[i] = (x, y, z,
+// k^2 m_i), and for the bodies that are sources of the indirect term (AC,
+// 1 <= i < end_max) sT[i] = r_i rm3_i.  rm3 = 1 / |r_i|^3 of the own bodies
+// (0 for the star and in BC) is returned for the Kepler term.  The barrier
+// makes both arrays visible to every sink.
 template <int BPT>
-__device__ void publish(const Ctx &c, const double (&pos)[BPT][3], const double (&gm)[BPT], double (&ind)[6])
+__device__ void publish(const Ctx &c, const double (&pos)[BPT][3], const double (&gm)[BPT], double (&rm3)[BPT])
 {
-    double v[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
         const int i = own(q);
+        rm3[q] = 0.0;
         if (i >= c.n) break;
         const double x = pos[q][0], y = pos[q][1], z = pos[q][2];
         c.sJ[i] = make_double4(x, y, z, gm[q]);
-        // only the bodies below the larger source end enter the indirect sums
-        if (!c.bc && i >= 1 && i < c.end_max) {
+        if (!c.bc && i >= 1) {
+            // Solaris/Acceleration.cpp:257-263
             const double r2 = x * x + y * y + z * z;
             const double r = sqrt(r2);
-            const double rm3 = 1.0 / (r2 * r);
-            const double cx = gm[q] * rm3 * x, cy = gm[q] * rm3 * y, cz = gm[q] * rm3 * z;
-            if (i < c.end_a) { v[0] += cx; v[1] += cy; v[2] += cz; }
-            if (i < c.end_b) { v[3] += cx; v[4] += cy; v[5] += cz; }
+            rm3[q] = 1.0 / (r2 * r);
+            if (i < c.end_max) c.sT[i] = make_double4(x * rm3[q], y * rm3[q], z * rm3[q], 0.0);
         }
     }
-    // Only the waves whose first body is below the source end hold sources
-    // (a wave's bodies are wave_first + q * blockDim); the others' partials
-    // are +0.0 and are left out.  The source waves reduce their partials and
-    // publish them; after the one barrier (which also publishes sJ) every
-    // thread adds the published partials in wave order itself.
-    const int nsw = c.bc ? 0 : min((int)(blockDim.x / kWave), (c.end_max + kWave - 1) / kWave);
-    const int wid = threadIdx.x / kWave;
-    if (wid < nsw) {
-        for (int k = 0; k < 6; ++k) v[k] = wsum(v[k]);
-        if ((threadIdx.x & (kWave - 1)) == 0)
-            for (int k = 0; k < 6; ++k) c.red[wid * 6 + k] = v[k];
-    }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        double s = nsw > 0 ? c.red[k] : 0.0;
-        for (int w = 1; w < nsw; ++w) s += c.red[w * 6 + k];
-        ind[k] = s;
-    }
 }
 
-// Acceleration of body i from the LDS positions (GravityAC / GravityBC
-// semantics, Solaris/Acceleration.cpp:248-329, :530-634); mu = k^2 (m_0 +
-// m_i); nn: track the nearest source.
-__device__ void gravity_one(const Ctx &c, int i, double mu, const double (&ind)[6], bool nn, double &ax,
-                            double &ay, double &az, double &rm3_out, int &nni, double &nnd)
+// Pair sums of sink p over the sources j in [j0, j1) (which exclude the
+// sink itself).  AC (GravityAC, Solaris/Acceleration.cpp:294-317): over
+// the sources in order, k^2 m_j (d_ij / r_ij^3 - r_j rm3_j), the indirect
+// term taken pair by pair as the reference does.  BC (GravityBC,
+// :530-634): k^2 m_j d_ij / r_ij^3.  1 / r_ij^3 is the hardware reciprocal
+// square root cubed with one second-order correction,
+// (1 - e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2 for e = 1 - r^2 y^2.  NN: the
+// nearest source (BC: the last of equals, AC: the first).
+template <bool BC, bool NN>
+__device__ __forceinline__ void pair_range(const Ctx &c, const double4 &p, int j0, int j1, double &px, double &py,
+                                           double &pz, double &best, int &bidx)
 {
-    const double4 p = c.sJ[i];
-    double best = DBL_MAX;
-    int bidx = -1;
-    double px = 0.0, py = 0.0, pz = 0.0;
-    int jb, je;
-    if (c.bc) { jb = 0; je = c.nm; }
-    else { jb = 1; je = (i < c.nm) ? c.end_a : c.end_b; }
-    if (!c.bc && i == 0) je = jb;   // the star is not integrated (AC)
-    for (int j = jb; j < je; ++j) {
-        if (j == i) continue;
+#pragma unroll 2
+    for (int j = j0; j < j1; ++j) {
         const double4 q = c.sJ[j];
+        double4 t;
+        if constexpr (!BC) t = c.sT[j];
         const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
         const double r2 = dx * dx + dy * dy + dz * dz;
         const double y0 = __builtin_amdgcn_rsq(r2);
         const double tt = y0 * y0;
         const double e = __builtin_fma(-r2, tt, 1.0);
         const double y3 = tt * y0;
-        const double mq = q.w * y3;
-        const double f = __builtin_fma(mq, e * __builtin_fma(1.875, e, 1.5), mq);
-        px = __builtin_fma(f, dx, px);
-        py = __builtin_fma(f, dy, py);
-        pz = __builtin_fma(f, dz, pz);
-        if (nn) {
-            if (c.bc ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
+        const double rijm3 = __builtin_fma(y3, e * __builtin_fma(1.875, e, 1.5), y3);
+        if constexpr (BC) {
+            const double f = q.w * rijm3;
+            px = __builtin_fma(f, dx, px);
+            py = __builtin_fma(f, dy, py);
+            pz = __builtin_fma(f, dz, pz);
+        } else {
+            px = __builtin_fma(q.w, __builtin_fma(dx, rijm3, -t.x), px);
+            py = __builtin_fma(q.w, __builtin_fma(dy, rijm3, -t.y), py);
+            pz = __builtin_fma(q.w, __builtin_fma(dz, rijm3, -t.z), pz);
+        }
+        if constexpr (NN) {
+            if (BC ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
         }
     }
-    rm3_out = 0.0;
-    if (c.bc) {
+}
+
+// Acceleration of body i from the LDS positions; mu = k^2 (m_0 + m_i),
+// rm3 = 1 / |r_i|^3.  AC: the Kepler term -mu rm3 r_i plus the pair sum
+// over the sources 1 <= j < end (end by the sink's class,
+// Solaris/Acceleration.cpp:285-289); the star gets zero.  BC: the pair sum
+// over the massive bodies.
+template <bool BC, bool NN>
+__device__ void gravity_one(const Ctx &c, int i, double mu, double rm3, double &ax, double &ay, double &az,
+                            int &nni, double &nnd)
+{
+    const double4 p = c.sJ[i];
+    double best = DBL_MAX;
+    int bidx = -1;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    const int jb = BC ? 0 : 1;
+    int je = BC ? c.nm : (i < c.nm) ? c.end_a : c.end_b;
+    if (!BC && i == 0) je = jb;   // the star is not integrated (AC)
+    pair_range<BC, NN>(c, p, jb, min(i, je), px, py, pz, best, bidx);
+    pair_range<BC, NN>(c, p, max(i + 1, jb), je, px, py, pz, best, bidx);
+    if (BC) {
         ax = px; ay = py; az = pz;
     } else if (i == 0) {
         ax = ay = az = 0.0;
     } else {
-        const double r2 = p.x * p.x + p.y * p.y + p.z * p.z;
-        const double r = sqrt(r2);
-        const double rm3 = 1.0 / (r2 * r);
-        rm3_out = rm3;
-        const int end = (i < c.nm) ? c.end_a : c.end_b;
-        const int sgi = (i < c.nm) ? 0 : 3;
-        double ix = ind[sgi], iy = ind[sgi + 1], iz = ind[sgi + 2];
-        if (i < end) { ix -= p.w * rm3 * p.x; iy -= p.w * rm3 * p.y; iz -= p.w * rm3 * p.z; }
-        ax = -mu * rm3 * p.x + (px - ix);
-        ay = -mu * rm3 * p.y + (py - iy);
-        az = -mu * rm3 * p.z + (pz - iz);
+        ax = -mu * rm3 * p.x + px;
+        ay = -mu * rm3 * p.y + py;
+        az = -mu * rm3 * p.z + pz;
     }
     nni = bidx;
     nnd = bidx >= 0 ? sqrt(best) : 0.0;
 }
 
-// Evaluates the own bodies into stage slot l: the velocity half from vel,
-// the acceleration into acc (and slot l); the last stage also writes the
-// side outputs.  Ends with the barrier after which sJ may be overwritten.
-template <int BPT>
-__device__ void evaluate(const Ctx &c, int l, bool last, const double (&vel)[BPT][3], const double (&mu)[BPT],
-                         const double (&ind)[6], double (&acc)[BPT][3])
+// Evaluates the own bodies into stage slot l and into f (velocity half from
+// vel, zero for the astrocentric star; acceleration half from gravity); the
+// last stage also writes the side outputs.  Ends with the barrier after
+// which sJ and sT may be overwritten.
+template <int BPT, bool BC, bool LAST>
+__device__ void evaluate_t(const Ctx &c, int l, const double (&vel)[BPT][3], const double (&mu)[BPT],
+                           const double (&rm3)[BPT], double (&f)[BPT][6])
 {
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
         const int i = own(q);
         if (i >= c.n) break;
-        double ax, ay, az, rm3, nnd;
+        double ax, ay, az, nnd;
         int nni;
-        gravity_one(c, i, mu[q], ind, last, ax, ay, az, rm3, nni, nnd);
-        const bool star_ac = !c.bc && i == 0;
+        gravity_one<BC, LAST>(c, i, mu[q], rm3[q], ax, ay, az, nni, nnd);
+        const bool star_ac = !BC && i == 0;
+        f[q][0] = star_ac ? 0.0 : vel[q][0];
+        f[q][1] = star_ac ? 0.0 : vel[q][1];
+        f[q][2] = star_ac ? 0.0 : vel[q][2];
+        f[q][3] = ax; f[q][4] = ay; f[q][5] = az;
         uint32_t vb[6];
         offsets(c, i, vb);
-        double *f = stage(c, l);
-        st(f, vb[0], star_ac ? 0.0 : vel[q][0]);
-        st(f, vb[1], star_ac ? 0.0 : vel[q][1]);
-        st(f, vb[2], star_ac ? 0.0 : vel[q][2]);
-        st(f, vb[3], ax);
-        st(f, vb[4], ay);
-        st(f, vb[5], az);
-        acc[q][0] = ax; acc[q][1] = ay; acc[q][2] = az;
-        if (last) {
-            if (!c.bc) c.A->rm3[c.base + i] = rm3;
+        double *fs = stage(c, l);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) st(fs, vb[k], f[q][k]);
+        if (LAST) {
+            if (!BC) c.A->rm3[c.base + i] = rm3[q];
             c.A->nni[c.base + i] = nni;
             c.A->nnd[c.base + i] = nnd;
         }
@@ -257,10 +255,40 @@ __device__ void evaluate(const Ctx &c, int l, bool last, const double (&vel)[BPT
     __syncthreads();
 }
 
+template <int BPT>
+__device__ __forceinline__ void evaluate(const Ctx &c, int l, bool last, const double (&vel)[BPT][3],
+                                         const double (&mu)[BPT], const double (&rm3)[BPT], double (&f)[BPT][6])
+{
+    if (c.bc) {
+        if (last) evaluate_t<BPT, true, true>(c, l, vel, mu, rm3, f);
+        else evaluate_t<BPT, true, false>(c, l, vel, mu, rm3, f);
+    } else {
+        if (last) evaluate_t<BPT, false, true>(c, l, vel, mu, rm3, f);
+        else evaluate_t<BPT, false, false>(c, l, vel, mu, rm3, f);
+    }
+}
+
+// With one body per thread the state y0 and the previous stage's
+// derivative stay in registers (Reg); with more, both are read back from
+// memory.
+template <int BPT>
+struct Reg {
+    static constexpr bool on = BPT == 1;
+    double y0[BPT][6], prev[BPT][6];
+};
+
+// Stage derivative L of own body q, component k (at byte offset o).
+template <int L, int S, int BPT>
+__device__ __forceinline__ double fget(const Ctx &c, const Reg<BPT> &R, int q, int k, uint32_t o)
+{
+    if constexpr (Reg<BPT>::on && L == S - 1 && L > 0) return R.prev[q][k];
+    else return ld(stage(c, L), o);
+}
+
 // State of stage S (the reference's yTemp before the S-th evaluation) for
 // the own bodies: positions into pos, velocities into vel.  The stage
-// derivatives with nonzero coefficients and y0 are read first, all
-// together, so that their latencies overlap.
+// derivatives with nonzero coefficients and y0 are read first, three
+// components at a time, so that their latencies overlap.
 //
 // First-order tableaux (RKF78, RK4): y0 + h * sum, the sum over the nonzero
 // coefficients in the tableau's order, as the reference's hand-written
@@ -269,7 +297,8 @@ __device__ void evaluate(const Ctx &c, int l, bool last, const double (&vel)[BPT
 // p0 + c_S h v0 + h^2 var and velocities v0 + h var, var = sum_l a_{S,l} F_l
 // over the acceleration halves.
 template <int INTEG, int S, int BPT>
-__device__ __forceinline__ void stage_state(const Ctx &c, double h, double (&pos)[BPT][3], double (&vel)[BPT][3])
+__device__ __forceinline__ void stage_state(const Ctx &c, const Reg<BPT> &R, double h, double (&pos)[BPT][3],
+                                            double (&vel)[BPT][3])
 {
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
@@ -277,14 +306,15 @@ __device__ __forceinline__ void stage_state(const Ctx &c, double h, double (&pos
         if (i >= c.n) break;
         uint32_t vb[6];
         offsets(c, i, vb);
+        auto y0 = [&](int k) { return Reg<BPT>::on ? R.y0[q][k] : ld(c.A->y0, vb[k]); };
         if constexpr (INTEG == INTEG_DP) {
             double p0[3], v0[3], f[3][S];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                p0[j] = ld(c.A->y0, vb[j]);
-                v0[j] = ld(c.A->y0, vb[j + 3]);
+                p0[j] = y0(j);
+                v0[j] = y0(j + 3);
                 static_for<0, S>([&](auto L) {
-                    if constexpr (L == 0 || tab::kDpA[S][L] != 0.0) f[j][L] = ld(stage(c, L), vb[j + 3]);
+                    if constexpr (L == 0 || tab::kDpA[S][L] != 0.0) f[j][L] = fget<L, S>(c, R, q, j + 3, vb[j + 3]);
                 });
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -301,13 +331,13 @@ __device__ __forceinline__ void stage_state(const Ctx &c, double h, double (&pos
             // positions, then velocities: three components' loads at a time
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
-                double y0[3], f[3][S];
+                double yv[3], f[3][S];
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
-                    const uint32_t o = vb[3 * half + kk];
-                    y0[kk] = ld(c.A->y0, o);
+                    const int k = 3 * half + kk;
+                    yv[kk] = y0(k);
                     static_for<0, S>([&](auto L) {
-                        if constexpr (dcoef<INTEG>(S, L) != 0.0) f[kk][L] = ld(stage(c, L), o);
+                        if constexpr (dcoef<INTEG>(S, L) != 0.0) f[kk][L] = fget<L, S>(c, R, q, k, vb[k]);
                     });
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -323,7 +353,7 @@ __device__ __forceinline__ void stage_state(const Ctx &c, double h, double (&pos
                             first = false;
                         }
                     });
-                    const double out = y0[kk] + h * acc;
+                    const double out = yv[kk] + h * acc;
                     if (half == 0) pos[q][kk] = out;
                     else vel[q][kk] = out;
                 }
@@ -334,21 +364,22 @@ __device__ __forceinline__ void stage_state(const Ctx &c, double h, double (&pos
 
 // stage_state for the run-time stage number s (1 <= s <= n_stages).
 template <int INTEG, int BPT, int S = 1>
-__device__ __forceinline__ void stage_state_at(int s, const Ctx &c, double h, double (&pos)[BPT][3],
-                                               double (&vel)[BPT][3])
+__device__ __forceinline__ void stage_state_at(int s, const Ctx &c, const Reg<BPT> &R, double h,
+                                               double (&pos)[BPT][3], double (&vel)[BPT][3])
 {
     if constexpr (S <= n_stages<INTEG>()) {
-        if (s == S) stage_state<INTEG, S, BPT>(c, h, pos, vel);
-        else stage_state_at<INTEG, BPT, S + 1>(s, c, h, pos, vel);
+        if (s == S) stage_state<INTEG, S, BPT>(c, R, h, pos, vel);
+        else stage_state_at<INTEG, BPT, S + 1>(s, c, R, h, pos, vel);
     }
 }
 
-// Final state into Y and the error estimate into ERR for the own bodies;
-// returns this thread's contribution to the error maximum (RKF78:
-// max |err / yscale|, Dormand-Prince: max position error).
+// Final state into Y (and into ynew) and the error estimate into ERR for
+// the own bodies; returns this thread's contribution to the error maximum
+// (RKF78: max |err / yscale|, Dormand-Prince: max position error).
 template <int INTEG, int BPT>
-__device__ double final_state(const Ctx &c, double h)
+__device__ double final_state(const Ctx &c, const Reg<BPT> &R, double h, double (&ynew)[BPT][6])
 {
+    constexpr int NS = n_stages<INTEG>() + 1;
     double ratio = 0.0;
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
@@ -356,68 +387,71 @@ __device__ double final_state(const Ctx &c, double h)
         if (i >= c.n) break;
         uint32_t vb[6];
         offsets(c, i, vb);
-        const double *y0 = c.A->y0;
+        auto y0 = [&](int k) { return Reg<BPT>::on ? R.y0[q][k] : ld(c.A->y0, vb[k]); };
         if constexpr (INTEG == INTEG_RKF78) {
             // RungeKuttaFehlberg78.cpp:231-248
             constexpr double b0 = 41.0 / 840.0, b5 = 34.0 / 105.0, b6 = 9.0 / 35.0, b8 = 9.0 / 280.0,
                              b10 = 41.0 / 840.0;
-            double y[6], e[6];
+            double e[6];
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
-                double f[3][11], y0v[3], sc[3];
+                double f[3][13], yv[3], sc[3];
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
-                    const uint32_t o = vb[3 * half + kk];
-                    f[kk][0] = ld(stage(c, 0), o);
-#pragma unroll
-                    for (int l = 5; l <= 12; ++l) f[kk][l - 4] = ld(stage(c, l), o);
-                    y0v[kk] = ld(y0, o);
+                    const int k = 3 * half + kk;
+                    const uint32_t o = vb[k];
+                    f[kk][0] = fget<0, NS>(c, R, q, k, o);
+                    static_for<5, 13>([&](auto L) { f[kk][L] = fget<L, NS>(c, R, q, k, o); });
+                    yv[kk] = y0(k);
                     sc[kk] = ld(c.A->yscale, o);
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
                     const int k = 3 * half + kk;
-                    const double F0 = f[kk][0], F5 = f[kk][1], F6 = f[kk][2], F7 = f[kk][3], F8 = f[kk][4];
-                    const double F9 = f[kk][5], F10 = f[kk][6], F11 = f[kk][7], F12 = f[kk][8];
-                    y[k] = y0v[kk] + h * (b0 * F0 + b5 * F5 + b6 * (F6 + F7) + b8 * (F8 + F9) + b10 * F10);
-                    e[k] = h * fabs(F0 + F10 - F11 - F12) * 41.0 / 840.0;
+                    const double *F = f[kk];
+                    ynew[q][k] = yv[kk] + h * (b0 * F[0] + b5 * F[5] + b6 * (F[6] + F[7]) + b8 * (F[8] + F[9]) +
+                                               b10 * F[10]);
+                    e[k] = h * fabs(F[0] + F[10] - F[11] - F[12]) * 41.0 / 840.0;
                     ratio = fmax(ratio, fabs(e[k] / sc[kk]));
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 6; ++k) { st(c.A->y, vb[k], y[k]); st(c.A->err, vb[k], e[k]); }
+            for (int k = 0; k < 6; ++k) { st(c.A->y, vb[k], ynew[q][k]); st(c.A->err, vb[k], e[k]); }
         } else if constexpr (INTEG == INTEG_RK4) {
-            double y[6];
+            double f[6][4];
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                static_for<0, 4>([&](auto L) { f[k][L] = fget<L, NS>(c, R, q, k, vb[k]); });
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                const uint32_t o = vb[k];
-                y[k] = ld(y0, o) + h * (tab::kRk4B[0] * ld(stage(c, 0), o) + tab::kRk4B[1] * ld(stage(c, 1), o) +
-                                        tab::kRk4B[2] * ld(stage(c, 2), o) + tab::kRk4B[3] * ld(stage(c, 3), o));
+                ynew[q][k] = y0(k) + h * (tab::kRk4B[0] * f[k][0] + tab::kRk4B[1] * f[k][1] +
+                                          tab::kRk4B[2] * f[k][2] + tab::kRk4B[3] * f[k][3]);
+                st(c.A->y, vb[k], ynew[q][k]);
             }
-#pragma unroll
-            for (int k = 0; k < 6; ++k) st(c.A->y, vb[k], y[k]);
         } else {
             // DormandPrince.cpp:455-491
             constexpr const double *B = tab::kDpB, *Bd = tab::kDpBdh;
             const double h2 = h * h;
-            double p[3], v[3], e[3];
+            double e[3];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const uint32_t o = vb[j + 3];
-                const double F0 = ld(stage(c, 0), o), F4 = ld(stage(c, 4), o), F5 = ld(stage(c, 5), o);
-                const double F6 = ld(stage(c, 6), o), F7 = ld(stage(c, 7), o), F8 = ld(stage(c, 8), o);
-                const double p0 = ld(y0, vb[j]), v0 = ld(y0, o);
-                p[j] = p0 + h * v0 + h2 * (B[0] * F0 + B[4] * F4 + B[5] * F5 + B[6] * F6 + B[7] * F7 + B[8] * F8);
-                e[j] = h2 * fabs(F7 - F8) / 20.0;
-                v[j] = v0 + h * (Bd[0] * F0 + Bd[4] * F4 + Bd[5] * F5 + Bd[6] * F6 + Bd[7] * F7);
+                double F[9];
+                F[0] = fget<0, NS>(c, R, q, j + 3, o);
+                static_for<4, 9>([&](auto L) { F[L] = fget<L, NS>(c, R, q, j + 3, o); });
+                const double p0 = y0(j), v0 = y0(j + 3);
+                ynew[q][j] = p0 + h * v0 + h2 * (B[0] * F[0] + B[4] * F[4] + B[5] * F[5] + B[6] * F[6] +
+                                                 B[7] * F[7] + B[8] * F[8]);
+                e[j] = h2 * fabs(F[7] - F[8]) / 20.0;
+                ynew[q][j + 3] = v0 + h * (Bd[0] * F[0] + Bd[4] * F[4] + Bd[5] * F[5] + Bd[6] * F[6] + Bd[7] * F[7]);
                 ratio = fmax(ratio, e[j]);
             }
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                st(c.A->y, vb[j], p[j]);
+                st(c.A->y, vb[j], ynew[q][j]);
                 st(c.A->err, vb[j], e[j]);
-                st(c.A->y, vb[j + 3], v[j]);
+                st(c.A->y, vb[j + 3], ynew[q][j + 3]);
                 st(c.A->err, vb[j + 3], 0.0);
             }
         }
@@ -426,12 +460,15 @@ __device__ double final_state(const Ctx &c, double h)
 }
 
 // One workgroup per system; BPT bodies per thread at most
-// (n <= BPT * blockDim.x); blocks of BPT = 4 have at most 512 threads.
+// (n <= BPT * blockDim.x).  Blocks have at most 640 threads with one body
+// per thread (168 registers) and 512 with more (256 registers).
+// Dynamic LDS: sJ and sT, nmax records each.
 template <int INTEG, int BPT>
-__global__ __launch_bounds__(BPT >= 4 ? 512 : 1024) void k_resident(ResSystem *systems, ResArrays A, int max_steps)
+__global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *systems, ResArrays A, int max_steps,
+                                                                     int nmax)
 {
-    extern __shared__ double4 sJ[];
-    __shared__ double red[16 * 6];
+    extern __shared__ double4 lds[];
+    __shared__ double red[16];
     __shared__ double sh[4];
     ResSystem &S = systems[blockIdx.x];
     if (S.status != RES_RUNNING) return;
@@ -439,8 +476,9 @@ __global__ __launch_bounds__(BPT >= 4 ? 512 : 1024) void k_resident(ResSystem *s
     c.A = &A; c.n = S.n; c.base = S.base; c.nm = S.nm; c.end_a = S.end_a; c.end_b = S.end_b;
     c.end_max = c.end_a > c.end_b ? c.end_a : c.end_b;
     c.bc = S.barycentric != 0;
-    c.tot = A.tot; c.sJ = sJ; c.red = red;
+    c.tot = A.tot; c.sJ = lds; c.sT = lds + nmax;
     constexpr int nst = n_stages<INTEG>();
+    constexpr bool kReg = Reg<BPT>::on;
 
     // k^2 m_i and k^2 (m_0 + m_i) of the own bodies, for the whole launch
     double gm[BPT], mu[BPT];
@@ -450,53 +488,71 @@ __global__ __launch_bounds__(BPT >= 4 ? 512 : 1024) void k_resident(ResSystem *s
         gm[q] = kGauss2 * A.mass[c.base + i];
         mu[q] = kGauss2 * (A.mass[c.base] + A.mass[c.base + i]);
     }
+    Reg<BPT> R;
+    if constexpr (kReg) {
+#pragma unroll
+        for (int q = 0; q < BPT; ++q) {
+            uint32_t vb[6];
+            offsets(c, own(q) < c.n ? own(q) : 0, vb);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) R.y0[q][k] = ld(A.y0, vb[k]);
+        }
+    }
 
     for (int step = 0; step < max_steps; ++step) {
         const double t0 = S.t;
         double h = S.h_next;
         // f0 at y0 (the Driver's first evaluation), and yscale
         {
-            double y0[BPT][6], pos[BPT][3], vel[BPT][3], acc[BPT][3], ind[6];
+            double y0[BPT][6], pos[BPT][3], vel[BPT][3], rm3[BPT], f[BPT][6];
 #pragma unroll
             for (int q = 0; q < BPT; ++q) {
-                uint32_t vb[6];
-                offsets(c, own(q) < c.n ? own(q) : 0, vb);
+                if constexpr (kReg) {
 #pragma unroll
-                for (int k = 0; k < 6; ++k) y0[q][k] = ld(A.y0, vb[k]);
+                    for (int k = 0; k < 6; ++k) y0[q][k] = R.y0[q][k];
+                } else {
+                    uint32_t vb[6];
+                    offsets(c, own(q) < c.n ? own(q) : 0, vb);
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) y0[q][k] = ld(A.y0, vb[k]);
+                }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) { pos[q][k] = y0[q][k]; vel[q][k] = y0[q][k + 3]; }
             }
-            publish<BPT>(c, pos, gm, ind);
-            evaluate<BPT>(c, 0, false, vel, mu, ind, acc);
+            publish<BPT>(c, pos, gm, rm3);
+            evaluate<BPT>(c, 0, false, vel, mu, rm3, f);
             if (INTEG == INTEG_RKF78) {
 #pragma unroll
                 for (int q = 0; q < BPT; ++q) {
                     const int i = own(q);
                     if (i >= c.n) break;
-                    const bool star_ac = !c.bc && i == 0;
                     uint32_t vb[6];
                     offsets(c, i, vb);
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) {
-                        const double f = star_ac ? 0.0 : k < 3 ? vel[q][k] : acc[q][k - 3];
-                        st(A.yscale, vb[k], fabs(y0[q][k]) + fabs(h * f) + 1.0e-30);
-                    }
+                    for (int k = 0; k < 6; ++k) st(A.yscale, vb[k], fabs(y0[q][k]) + fabs(h * f[q][k]) + 1.0e-30);
                 }
             }
         }
         int iter = 0;
         double errmax = 0.0;
         int outcome = 0;   // 0 accept, 1 retry, 2 underflow, 3 maxiter
+        double ynew[BPT][6];
         for (;;) {
             ++iter;
             if (INTEG == INTEG_DP) h = S.h_next;
             for (int s = 1; s <= nst; ++s) {
-                double pos[BPT][3], vel[BPT][3], acc[BPT][3], ind[6];
-                stage_state_at<INTEG, BPT>(s, c, h, pos, vel);
-                publish<BPT>(c, pos, gm, ind);
-                evaluate<BPT>(c, s, s == nst, vel, mu, ind, acc);
+                double pos[BPT][3], vel[BPT][3], rm3[BPT], f[BPT][6];
+                stage_state_at<INTEG, BPT>(s, c, R, h, pos, vel);
+                publish<BPT>(c, pos, gm, rm3);
+                evaluate<BPT>(c, s, s == nst, vel, mu, rm3, f);
+                if constexpr (kReg) {
+#pragma unroll
+                    for (int q = 0; q < BPT; ++q)
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) R.prev[q][k] = f[q][k];
+                }
             }
-            const double ratio = final_state<INTEG, BPT>(c, h);
+            const double ratio = final_state<INTEG, BPT>(c, R, h, ynew);
             const double raw = block_max_all(ratio, red, &sh[0]);
             if (threadIdx.x == 0) S.attempts += 1;
             // the step decision (identical in every thread)
@@ -535,18 +591,22 @@ __global__ __launch_bounds__(BPT >= 4 ? 512 : 1024) void k_resident(ResSystem *s
             h_next = S.h_next;
             h = S.h_did;
         }
-        // swap y0 and y (contents), own bodies
+        // swap y0 and y (the Driver's pointer swap), own bodies
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
             const int i = own(q);
             if (i >= c.n) break;
             uint32_t vb[6];
             offsets(c, i, vb);
-            double a[6], b[6];
+            double old[6];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) { a[k] = ld(A.y0, vb[k]); b[k] = ld(A.y, vb[k]); }
+            for (int k = 0; k < 6; ++k) old[k] = kReg ? R.y0[q][k] : ld(A.y0, vb[k]);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) { st(A.y0, vb[k], b[k]); st(A.y, vb[k], a[k]); }
+            for (int k = 0; k < 6; ++k) { st(A.y0, vb[k], ynew[q][k]); st(A.y, vb[k], old[k]); }
+            if constexpr (kReg) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) R.y0[q][k] = ynew[q][k];
+            }
         }
         __syncthreads();
         // CheckEvent's thresholds on the last stage's side outputs
@@ -606,6 +666,13 @@ __global__ __launch_bounds__(BPT >= 4 ? 512 : 1024) void k_resident(ResSystem *s
                     if (fabs(cat(A.y0, c, k, i)) < 1.0e-50) at(A.y0, c, k, i) = 0.0;
                     if (fabs(cat(A.y, c, k, i)) < 1.0e-50) at(A.y, c, k, i) = 0.0;
                 }
+            if constexpr (kReg) {
+#pragma unroll
+                for (int q = 0; q < BPT; ++q)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k)
+                        if (fabs(R.y0[q][k]) < 1.0e-50) R.y0[q][k] = 0.0;
+            }
         }
         __syncthreads();
         if (next != 0.0) {
@@ -619,44 +686,45 @@ __global__ __launch_bounds__(BPT >= 4 ? 512 : 1024) void k_resident(ResSystem *s
 
 int resident_block_size(int n)
 {
+    if (n > 640) { const int bpt = n <= 1024 ? 2 : 4; n = (n + bpt - 1) / bpt; }
     int b = ((n + kWave - 1) / kWave) * kWave;
     if (b < kWave) b = kWave;
     if (b > 1024) b = 1024;
     return b;
 }
 
-size_t resident_lds_bytes(int nmax) { return (size_t)nmax * sizeof(double4); }
+size_t resident_lds_bytes(int nmax) { return 2 * (size_t)nmax * sizeof(double4); }
 
 namespace {
 
 template <int INTEG>
 void launch_bpt(hipStream_t st, ResSystem *systems, int nsys, const ResArrays &A, int max_steps, int block,
-                int bpt, size_t lds)
+                int bpt, int nmax, size_t lds)
 {
     switch (bpt) {
-    case 1: hipLaunchKernelGGL((k_resident<INTEG, 1>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps); break;
-    case 2: hipLaunchKernelGGL((k_resident<INTEG, 2>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps); break;
-    default: hipLaunchKernelGGL((k_resident<INTEG, 4>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps); break;
+    case 1: hipLaunchKernelGGL((k_resident<INTEG, 1>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps, nmax); break;
+    case 2: hipLaunchKernelGGL((k_resident<INTEG, 2>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps, nmax); break;
+    default: hipLaunchKernelGGL((k_resident<INTEG, 4>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps, nmax); break;
     }
 }
 
 }  // namespace
 
-// Bodies per thread: the smallest of 1, 2, 4 that covers nmax.  A block too
-// small for 4 per thread is widened to the smallest that is not, and one of
-// more than 512 threads that would need 4 to 1024 threads (2 per thread
-// cover kResidentMaxBodies).
+// Bodies per thread: the smallest of 1, 2, 4 that covers nmax.  A block of
+// more than 640 threads, or of more than 512 that does not cover nmax, is
+// narrowed to 512; one too small for 4 per thread is widened to the
+// smallest that is not.
 void launch_resident(hipStream_t st, int integ, ResSystem *systems, int nsys, const ResArrays &A, int max_steps,
                      int block, int nmax)
 {
     const size_t lds = resident_lds_bytes(nmax);
+    if (block > 640 || (block > 512 && nmax > block)) block = 512;
     if (block * 4 < nmax) block = ((nmax + 3) / 4 + kWave - 1) / kWave * kWave;
-    if (block > 512 && nmax > 2 * block) block = 1024;
     const int bpt = nmax <= block ? 1 : nmax <= 2 * block ? 2 : 4;
     switch (integ) {
-    case INTEG_RKF78: launch_bpt<INTEG_RKF78>(st, systems, nsys, A, max_steps, block, bpt, lds); break;
-    case INTEG_RK4: launch_bpt<INTEG_RK4>(st, systems, nsys, A, max_steps, block, bpt, lds); break;
-    default: launch_bpt<INTEG_DP>(st, systems, nsys, A, max_steps, block, bpt, lds); break;
+    case INTEG_RKF78: launch_bpt<INTEG_RKF78>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds); break;
+    case INTEG_RK4: launch_bpt<INTEG_RK4>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds); break;
+    default: launch_bpt<INTEG_DP>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds); break;
     }
 }
 
