@@ -242,9 +242,9 @@ def traffic_probe(n: int):
     (FETCH_SIZE, then WRITE_SIZE) that give the pair kernel's HBM traffic:
     a plain device-to-device copy of known size (2 GiB read, 2 GiB
     written), whose counter readings calibrate the counters, then one
-    accepted RKF78 step of the benchmark workload (per attempt, 12
-    launches of the symmetric pair kernel and one of the one-directional
-    kernel with the nearest-neighbour search)."""
+    accepted RKF78 step of the benchmark workload (per attempt, 13
+    launches of the symmetric pair kernel, 13 epilogues and, at the last
+    stage, the grid nearest-neighbour search)."""
     import torch
     from solaris_amd import _abi, runtime, scenarios as S
     src = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
@@ -345,8 +345,8 @@ def main():
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": load_traffic(kernel),
                      "avg_launch_ms": pair_ms, "launches": launches,
-                     "nn_variant_avg_launch_ms": g["pair_nn_ms"] / max(g["pair_nn_launches"], 1),
-                     "nn_variant_launches": g["pair_nn_launches"],
+                     "nn_search_ms_per_launch": g["pair_nn_ms"] / max(g["pair_nn_launches"], 1),
+                     "nn_search_launches": g["pair_nn_launches"],
                      "pair_interactions_per_launch": pairs_per_launch,
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "epilogue_ms_per_launch": g["epilogue_ms"] / max(g["epilogue_launches"], 1),

@@ -171,6 +171,21 @@ void build_plan(sol_context *c)
 
 int ensure_partials(sol_context *c)
 {
+    if (c->sym.M > 0 && !c->grid.params) {
+        // the nearest-neighbour search that goes with the symmetric sum
+        NNGrid &g = c->grid;
+        const size_t mc = nn_grid_max_cells(c->n), np = (size_t)c->np;
+        HIPCHECK(c, hipMalloc(&g.params, sizeof(GridParams)));
+        HIPCHECK(c, hipMalloc(&g.bbox, kGridBboxBlocks * 6 * sizeof(double)));
+        HIPCHECK(c, hipMalloc(&g.count, (mc + 1) * sizeof(int32_t)));
+        HIPCHECK(c, hipMalloc(&g.start, (mc + 1) * sizeof(int32_t)));
+        HIPCHECK(c, hipMalloc(&g.cellof, np * sizeof(int32_t)));
+        HIPCHECK(c, hipMalloc(&g.rank, np * sizeof(int32_t)));
+        HIPCHECK(c, hipMalloc(&g.spos, np * sizeof(double4)));
+        HIPCHECK(c, hipMalloc(&g.r2, np * sizeof(double)));
+        HIPCHECK(c, hipMalloc(&g.idx, np * sizeof(int32_t)));
+        g.max_cells = (int32_t)mc;
+    }
     if (c->sym.M > 0) {
         const size_t na = (size_t)c->sym.nitems * 3 * kSymGroup;
         const size_t nb = (size_t)c->sym.nwaves * c->sym.slots_per_wave * 3 * kSymGroup;
@@ -303,22 +318,17 @@ int upload_scalars(sol_context *c, double h, double t)
 // rest when the context has a symmetric plan (astrocentric, block of at
 // least 2,048 bodies), else the one-directional sum of everything.
 //
-// With nn (the last stage of an attempt, sol_compute with side outputs),
-// FP64 keeps the one-directional kernel, whose nearest-neighbour tracking
-// costs 0.5 ms on top of its 2.5 ms at 65,537 bodies, less than the
-// symmetric sum plus a separate search (1.6 + 1.6 ms).  Mixed precision
-// uses the symmetric sum plus the separate FP32 search: every stage then
-// sums its FP32 terms in the same kernel, so their rounding cancels in the
-// error estimate's F10 - F12 (RKF78) and F7 - F8 (Dormand-Prince), both
-// evaluated at the end of the step; with the one-directional kernel at
-// the last stage only, the estimate carried two unrelated FP32 roundings
-// and, on a velocity component near zero (small yscale), rejected steps
-// that FP64 accepts.  Sets the epilogue's plan fields.
+// With nn (the last stage of an attempt, sol_compute with side outputs) the
+// symmetric path adds the grid search for the nearest neighbours (FP64
+// positions, O(N)); the forces of every stage then come from the same
+// kernel, which in mixed precision also keeps the FP32 rounding of the
+// pair sums cancelling in the error estimate's F10 - F12 (RKF78) and
+// F7 - F8 (Dormand-Prince).  Sets the epilogue's plan fields.
 void pair_sums(sol_context *c, bool nn, EpiArgs &a)
 {
     const double eps2 = c->softening * c->softening;
     a.nn_sep = 0;
-    if (c->sym.M > 0 && (!nn || mixed_on(c))) {
+    if (c->sym.M > 0) {
         {
             ProfScope ps(c, 3, c->sym_inter_per_launch);
             if (mixed_on(c))
@@ -332,9 +342,10 @@ void pair_sums(sol_context *c, bool nn, EpiArgs &a)
                          c->J32);
         }
         if (nn) {
-            ProfScope ps(c, 2, c->pair_inter_per_launch);
-            launch_nn_only(c->stream, c->plan, c->J, c->J32, c->nnr2, c->nnidx, c->barycentric != 0, mixed_on(c));
-            a.nnplan = c->plan;
+            ProfScope ps(c, 2, 0.0);
+            launch_nn_grid(c->stream, c->grid, c->plan, c->J, c->n, c->barycentric != 0);
+            a.nnr2 = c->grid.r2;
+            a.nnidx = c->grid.idx;
             a.nn_sep = 1;
         }
         a.plan = c->plan_rest;
@@ -724,6 +735,12 @@ int sol_destroy(sol_context *c)
     if (c->nnidx) (void)hipFree(c->nnidx);
     if (c->apart) (void)hipFree(c->apart);
     if (c->bpart) (void)hipFree(c->bpart);
+    {
+        void *gp[] = {c->grid.params, c->grid.bbox, c->grid.count, c->grid.start, c->grid.cellof,
+                      c->grid.rank, c->grid.spos, c->grid.r2, c->grid.idx};
+        for (void *q : gp)
+            if (q) (void)hipFree(q);
+    }
     if (c->h_scal) (void)hipHostFree(c->h_scal);
     if (c->res_h) (void)hipHostFree(c->res_h);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);

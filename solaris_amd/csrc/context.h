@@ -42,6 +42,7 @@ struct sol_context {
     double *fst = nullptr, *cin = nullptr, *cout = nullptr, *aos = nullptr;
     double4 *J = nullptr;
     float4 *J32 = nullptr;     // allocated when the mixed form is selected
+    sol::NNGrid grid{};       // nearest-neighbour search of the symmetric path
     double *part = nullptr, *nnr2 = nullptr;
     int32_t *nnidx = nullptr;
     double *ind = nullptr;

@@ -632,94 +632,7 @@ __global__ __launch_bounds__(256, 4) void k_pairs_sym_mixed(SymPlan sp, const fl
     if (hcur >= 0) flush_col();
 }
 
-// Nearest-neighbour search alone (BodyData::indexOfNN / distanceOfNN of
-// GravityAC, Solaris/Acceleration.cpp:296-312, and GravityBC), for the
-// evaluations whose forces come from the symmetric sum: the same work
-// items, scan order, tie rule and slot layout as k_pairs_fp64<NN = true>,
-// without the force terms (3 subtractions, 3 operations for r^2 and the
-// compare per pair).  BC: ties go to the larger index; otherwise to the
-// smaller.  FP32: the search of k_pairs_mixed (single-precision r^2; the
-// epilogue recomputes the chosen distance in FP64).
-template <bool BC, bool FP32>
-__global__ __launch_bounds__(256) void k_nn_only(PairPlan plan, const double4 *__restrict__ J,
-                                                 const float4 *__restrict__ J32, double *__restrict__ nnr2,
-                                                 int32_t *__restrict__ nnidx)
-{
-    using real = typename std::conditional<FP32, float, double>::type;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = blockIdx.x * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    if (w >= plan.nwaves) return;
-    const int64_t lo = ((int64_t)w * plan.total_items) / plan.nwaves;
-    const int64_t hi = ((int64_t)(w + 1) * plan.total_items) / plan.nwaves;
-
-    int slot = 0, cur = -1;
-    int sb = 0, se = 0, i = 0;
-    real xi = 0, yi = 0, zi = 0;
-    real best = FP32 ? (real)FLT_MAX : (real)DBL_MAX;
-    int bidx = -1;
-    auto load = [&](int j, real &x, real &y, real &z) {
-        if (FP32) { const float4 q = J32[j]; x = q.x; y = q.y; z = q.z; }
-        else { const double4 q = J[j]; x = q.x; y = q.y; z = q.z; }
-    };
-    auto flush = [&](int k) {
-        const int64_t nb = ((int64_t)w * plan.slots_per_wave + k) * kWave;
-        nnr2[nb + lane] = (double)best;
-        nnidx[nb + lane] = bidx;
-    };
-    for (int64_t it = lo; it < hi; ++it) {
-        int s, t, c;
-        decode_item(plan, it, s, t, c);
-        const PairSeg &sg = plan.seg[s];
-        const int g = sg.tile_begin + t;
-        if (g != cur) {
-            if (cur >= 0) { flush(slot); ++slot; }
-            cur = g;
-            best = FP32 ? (real)FLT_MAX : (real)DBL_MAX;
-            bidx = -1;
-            sb = sg.sink_begin + t * kWave;
-            se = min(sb + kWave, sg.sink_end);
-            i = min(sb + lane, se - 1);      // spare lanes repeat the last sink
-            load(i, xi, yi, zi);
-        }
-        const int jb = sg.src_begin + c * plan.chunk;
-        const int je = min(jb + plan.chunk, sg.src_end);
-        if (jb >= se || sb >= je) {
-#pragma unroll 4
-            for (int j = jb; j < je; ++j) {
-                real x, y, z;
-                load(j, x, y, z);
-                const real dx = x - xi, dy = y - yi, dz = z - zi;
-                const real r2 = dx * dx + dy * dy + dz * dz;
-                if (BC ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
-            }
-        } else {
-            for (int j = jb; j < je; ++j) {
-                real x, y, z;
-                load(j, x, y, z);
-                const real dx = x - xi, dy = y - yi, dz = z - zi;
-                const real r2 = dx * dx + dy * dy + dz * dz;
-                if (j != i && (BC ? (r2 <= best) : (r2 < best))) { best = r2; bidx = j; }
-            }
-        }
-    }
-    if (cur >= 0) flush(slot);
-}
-
 }  // namespace
-
-void launch_nn_only(hipStream_t st, const PairPlan &plan, const double4 *J, const float4 *J32, double *nnr2,
-                    int32_t *nnidx, bool bc, bool fp32)
-{
-    if (plan.total_items == 0) return;
-    const int blocks = (plan.nwaves + (kBlock / kWave) - 1) / (kBlock / kWave);
-    if (fp32) {
-        if (bc) hipLaunchKernelGGL((k_nn_only<true, true>), dim3(blocks), dim3(kBlock), 0, st, plan, J, J32, nnr2, nnidx);
-        else hipLaunchKernelGGL((k_nn_only<false, true>), dim3(blocks), dim3(kBlock), 0, st, plan, J, J32, nnr2, nnidx);
-    } else {
-        if (bc) hipLaunchKernelGGL((k_nn_only<true, false>), dim3(blocks), dim3(kBlock), 0, st, plan, J, J32, nnr2, nnidx);
-        else hipLaunchKernelGGL((k_nn_only<false, false>), dim3(blocks), dim3(kBlock), 0, st, plan, J, J32, nnr2, nnidx);
-    }
-}
 
 void launch_pairs_sym_mixed(hipStream_t st, const SymPlan &sp, const float4 *J32, const double4 *J, double *apart,
                             double *bpart, double eps2)

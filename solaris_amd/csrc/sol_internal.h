@@ -98,8 +98,7 @@ struct EpiArgs {
     SymPlan sym;              // symmetric pair sum of the mutual block (used when use_sym)
     int32_t use_sym, pad3;
     const double *apart, *bpart;   // its row-side and column-side records
-    PairPlan nnplan;          // nearest-neighbour slots of a separate search (used when nn_sep)
-    int32_t nn_sep, pad4;
+    int32_t nn_sep, pad4;     // nearest neighbours: 0 in the pair partials, 1 per sink in nnr2/nnidx (grid search)
 };
 
 // Arguments of one force vector-Jacobian product (adjoint.hip).  Y is the
@@ -130,11 +129,34 @@ void fill_coefficients(std::vector<double> &c);
 void launch_pairs(hipStream_t st, const PairPlan &plan, const double4 *J, double *part,
                   double *nnr2, int32_t *nnidx, bool nn, bool barycentric, bool mixed,
                   double eps2, const float4 *J32);
-// Nearest-neighbour search alone over `plan` (slots of nnr2 / nnidx as the
-// NN variant of launch_pairs writes them); fp32: single-precision search
-// from J32.
-void launch_nn_only(hipStream_t st, const PairPlan &plan, const double4 *J, const float4 *J32, double *nnr2,
-                    int32_t *nnidx, bool bc, bool fp32);
+// Grid of the exact nearest-neighbour search (nngrid.hip): cubic cells of
+// side h from lo, dim[0] x dim[1] x dim[2] of them (x fastest).
+struct GridParams {
+    double lo[3];
+    double h, slack;
+    int32_t dim[3];
+    int32_t ncells;
+};
+
+// Device buffers of the search: params, bounding-box partials
+// [kGridBboxBlocks][6], counts and starts [max_cells + 1], per-source cell
+// and rank, the sources sorted by cell ((x, y, z, index) records), and the
+// result per sink: squared distance and index (-1: none).
+constexpr int kGridBboxBlocks = 256;
+struct NNGrid {
+    GridParams *params;
+    double *bbox;
+    int32_t *count, *start, *cellof, *rank;
+    double4 *spos;
+    double *r2;
+    int32_t *idx;
+    int32_t max_cells, pad;
+};
+size_t nn_grid_max_cells(int n);
+// Nearest neighbour of every sink of `plan` among its segment's sources
+// (itself excluded) from the positions J, into G.r2 / G.idx.
+void launch_nn_grid(hipStream_t st, const NNGrid &G, const PairPlan &plan, const double4 *J, int n, bool bc);
+
 // Symmetric pair sum over the mutual block (sym_plan.h): row-side records
 // in apart, column-side records in bpart.
 void launch_pairs_sym(hipStream_t st, const SymPlan &sp, const double4 *J, double *apart, double *bpart,

@@ -300,35 +300,6 @@ __device__ __forceinline__ int tile_of_item(const PairPlan &p, int64_t x)
     return p.seg[s].tile_begin + (int)((x - p.seg[s].item_begin) / p.seg[s].nchunks);
 }
 
-// Nearest neighbour of sink i from the slots of a separate search over the
-// plan q (the covering waves in order, as for the force partials).
-__device__ void gather_nn(const EpiArgs &a, const PairPlan &q, int i, double &best, int &bidx)
-{
-    for (int s = 0; s < q.nseg; ++s) {
-        const PairSeg &sg = q.seg[s];
-        if (i < sg.sink_begin || i >= sg.sink_end) continue;
-        const int t = (i - sg.sink_begin) / kWave;
-        const int lane = (i - sg.sink_begin) - t * kWave;
-        const int g = sg.tile_begin + t;
-        const int64_t A = sg.item_begin + (int64_t)t * sg.nchunks;
-        const int64_t B = A + sg.nchunks;
-        const int64_t w0 = wave_of_item(q, A), w1 = wave_of_item(q, B - 1);
-        for (int64_t w = w0; w <= w1; ++w) {
-            const int64_t lo = wave_lo(q, w), hi = wave_lo(q, w + 1);
-            if (lo >= hi || hi <= A || lo >= B) continue;
-            const int64_t slot = w * q.slots_per_wave + (g - tile_of_item(q, lo));
-            const double r2 = a.nnr2[slot * kWave + lane];
-            const int j = a.nnidx[slot * kWave + lane];
-            if (j >= 0) {
-                const bool take = a.barycentric ? (r2 < best || (r2 == best && j > bidx))
-                                                : (r2 < best || (r2 == best && (bidx < 0 || j < bidx)));
-                if (take) { best = r2; bidx = j; }
-            }
-        }
-        return;
-    }
-}
-
 // Sum of the pair partials for sink i; false if i is in no sink class.
 __device__ bool gather_pairs(const EpiArgs &a, int i, double &px, double &py, double &pz,
                              double &best, int &bidx)
@@ -337,7 +308,7 @@ __device__ bool gather_pairs(const EpiArgs &a, int i, double &px, double &py, do
     px = py = pz = 0.0;
     best = DBL_MAX; bidx = -1;
     bool found = false;
-    if (a.want_nn && a.nn_sep) gather_nn(a, a.nnplan, i, best, bidx);
+    if (a.want_nn && a.nn_sep && a.nnidx[i] >= 0) { best = a.nnr2[i]; bidx = a.nnidx[i]; }
     if (a.use_sym && i >= 1 && i < a.sym.M) {
         // symmetric sum of the mutual block: the row records of items (g, h),
         // h >= g, then the column records of the waves that cover column g

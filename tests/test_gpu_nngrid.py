@@ -1,0 +1,92 @@
+"""Nearest-neighbour side outputs of the symmetric path (the grid search of
+nngrid.hip) against the oracle's all-pairs scan
+(Solaris/Acceleration.cpp:296-312: indexOfNN / distanceOfNN of every sink
+over its type-based source range, ties to the smaller index).
+
+Indices must agree exactly, distances within 1e-14 relative.  The cases
+go where a grid search can go wrong:
+
+* exact ties: bodies placed on top of others, so that sinks see two
+  sources at the same distance and the pairs themselves are at distance 0;
+* a flat disk (all z = 0: one cell layer);
+* sinks outside the sources' box (test particles far beyond the disk) and
+  sinks whose source range is shorter than the grid's (super-planetesimals
+  and test particles only see the massive bodies), with few sources near
+  the outer ones so that the search has to widen far;
+* the full C3 disk at 65,537 bodies against the OpenMP oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle.cpu_ref import Oracle
+from solaris_amd import runtime, scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def check(scn, y=None, lib=None):
+    y = scn.y0 if y is None else y
+    ctx = runtime.Context(scn)
+    orc = Oracle(scn, lib=lib) if lib else Oracle(scn)
+    ctx.profile(True)
+    ctx.compute(0.0, y)
+    prof = ctx.profile(False)
+    orc.compute(0.0, y)
+    _, nn, d, _ = ctx.side_outputs()
+    _, nno, do, _ = orc.side_outputs()
+    bad = np.nonzero(nn != nno)[0]
+    print(f"[{scn.name} n={scn.n}] symmetric launches {prof.pair_sym_launches}, "
+          f"index mismatches {len(bad)} {bad[:5]}", flush=True)
+    assert prof.pair_sym_launches > 0, "the symmetric path (and with it the grid search) was not taken"
+    assert len(bad) == 0, (bad[:10], nn[bad[:10]], nno[bad[:10]])
+    np.testing.assert_allclose(d, do, rtol=1e-14, atol=0)
+    ctx.close()
+    orc.close()
+
+
+def test_ties_and_coincident_bodies():
+    scn = S.make_c3(n=4096)
+    y = scn.y0.copy()
+    # every 7th body onto its predecessor; some triples
+    k = np.arange(8, 4090, 7)
+    y[k, :3] = y[k - 1, :3]
+    y[k[::5] + 1, :3] = y[k[::5] - 1, :3]
+    check(scn, y)
+
+
+def test_flat_disk():
+    scn = S.make_c3(n=3000)
+    y = scn.y0.copy()
+    y[:, 2] = 0.0
+    y[:, 5] = 0.0
+    check(scn, y)
+
+
+def mixed_types(seed=0x501A00F1):
+    """Star, 2,500 protoplanets (0.5-5 AU), 500 super-planetesimals
+    (2-3 AU) and 500 test particles, 200 of them at 20-40 AU."""
+    rng = S.SplitMix64(seed)
+    groups = [S._star()]
+    n_pp, n_spl, n_tp = 2500, 500, 500
+    m = np.full(n_pp, 1.0e-9)
+    y = S._orbits(rng, n_pp, 1.0, m, rng.uniform(n_pp, 0.5, 5.0), rng.rayleigh(n_pp, 0.02),
+                  rng.rayleigh(n_pp, 0.01))
+    groups.append(dict(mass=m, type=S.PROTO, y0=y))
+    m = np.full(n_spl, 1.0e-10)
+    y = S._orbits(rng, n_spl, 1.0, m, rng.uniform(n_spl, 2.0, 3.0), rng.rayleigh(n_spl, 0.02),
+                  rng.rayleigh(n_spl, 0.01))
+    groups.append(dict(mass=m, type=S.SUPERPL, y0=y))
+    a = np.concatenate([rng.uniform(300, 0.5, 5.0), rng.uniform(200, 20.0, 40.0)])
+    y = S._orbits(rng, n_tp, 1.0, np.zeros(n_tp), a, rng.rayleigh(n_tp, 0.05), rng.rayleigh(n_tp, 0.3),
+                  massless=True)
+    groups.append(dict(mass=np.zeros(n_tp), type=S.TP, y0=y))
+    counts = S._counts(central_body=1, proto_planet=n_pp, super_planetesimal=n_spl, test_particle=n_tp)
+    return S._assemble("mixed_types", counts, groups, softening=0.0)
+
+
+def test_type_ranges_and_far_sinks():
+    check(mixed_types())
+
+
+def test_full_c3_against_openmp_oracle():
+    check(S.make_c3(), lib="omp")
