@@ -98,28 +98,39 @@ def test_sym_and_one_directional_paths_agree():
     assert err.max() <= 1e-14
 
 
-def test_sym_rkf78_replay():
+@pytest.mark.parametrize("integrator", [S.RUNGE_KUTTA_FEHLBERG78, S.RUNGE_KUTTA4, S.DORMAND_PRINCE])
+def test_sym_driver_replay(integrator):
+    """Each Driver through the symmetric path (2,601 bodies: past the
+    resident kernel, a symmetric block of 2,600) against the OpenMP oracle
+    on the oracle's step sequence: positions within 1e-9 AU over 10 steps,
+    and the last stage's side outputs (the grid search's nearest
+    neighbours) equal."""
     scn = S.make_c3(n=2600, softening=0.0)
     ctx = runtime.Context(scn)
     orc = Oracle(scn, lib="omp")
     ctx.set_state(scn.y0)
     orc.set_state(scn.y0)
-    to = _abi.timeline(0.0, scn.h_next, 1e12)
+    h0 = scn.h_next if integrator != S.RUNGE_KUTTA4 else 20.0 * scn.h_next
+    to = _abi.timeline(0.0, h0, 1e12)
     ctx.profile(True)
     worst = 0.0
     for k in range(10):
-        assert orc.driver(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, to)[0] == 0
+        assert orc.driver(integrator, 1e-10, to)[0] == 0
         tg = _abi.timeline(to.time - to.hDid, to.hDid, 1e12)
-        r, _ = ctx.driver(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, tg)
+        r, _ = ctx.driver(integrator, 1e-10, tg)
         assert r == 0, ctx.last_error()
         assert tg.hDid == to.hDid
         d = np.abs(ctx.get_array(runtime.Y0)[:, :3] - orc.get_array(runtime.Y0)[:, :3]).max()
         worst = max(worst, d)
     prof = ctx.profile(False)
-    print(f"  worst position difference over 10 steps: {worst:.3e} AU; symmetric launches {prof.pair_sym_launches}",
-          flush=True)
+    _, nn, dist, _ = ctx.side_outputs()
+    _, nno, disto, _ = orc.side_outputs()
+    print(f"  [integrator {integrator}] worst position difference over 10 steps: {worst:.3e} AU; "
+          f"symmetric launches {prof.pair_sym_launches}, NN mismatches {int((nn != nno).sum())}", flush=True)
     assert prof.pair_sym_launches > 0
     assert worst <= 1e-9
+    assert np.array_equal(nn, nno)
+    np.testing.assert_allclose(dist, disto, rtol=1e-12, atol=0)
 
 
 @pytest.mark.parametrize("n", [4096, 65536])
