@@ -128,9 +128,44 @@ def ensemble_throughput(nsys: int, days: float):
     wall = time.perf_counter() - t0
     done = sum(1 for k in range(nsys) if ens.get_system(k)[2] == 1)
     ens.close()
-    return dict(systems=nsys, span_days=days, systems_done=done, accepted_steps=int(st.accepted_steps),
-                interactions=st.interactions, wall_s=wall, interactions_per_s=st.interactions / wall,
-                system_steps_per_s=st.accepted_steps / wall, device_ms=st.device_ms)
+    out = dict(systems=nsys, span_days=days, systems_done=done, accepted_steps=int(st.accepted_steps),
+               interactions=st.interactions, wall_s=wall, interactions_per_s=st.interactions / wall,
+               system_steps_per_s=st.accepted_steps / wall, device_ms=st.device_ms)
+    cpu = ensemble_cpu_reference(scns[::64], days, 4.0)
+    if cpu:
+        out["cpu_reference"] = cpu
+        out["speedup_gpu_over_cpu"] = out["system_steps_per_s"] / cpu["system_steps_per_s"]
+    return out
+
+
+def ensemble_cpu_reference(scns, days: float, seconds: float):
+    """The compiled reference's RungeKuttaFehlberg78::Driver (oracle/_ref),
+    single thread, over a sample of the ensemble's systems, each from its
+    initial state towards `days` with the end-time clipping of the caller
+    loop: accepted system-steps per CPU second over a bounded time."""
+    from oracle import cpu_ref
+    from solaris_amd import _abi, scenarios as S
+    if not cpu_ref.available("ref"):
+        return None
+    steps = systems = 0
+    t0 = time.process_time()
+    for scn in scns:
+        if time.process_time() - t0 > seconds:
+            break
+        ref = cpu_ref.Reference(scn)
+        ref.set_state(scn.y0)
+        tl = _abi.timeline(0.0, scn.h_next, days)
+        while tl.time < days and time.process_time() - t0 <= seconds:
+            if tl.time + tl.hNext > days:
+                tl.hNext = days - tl.time
+            if ref.driver(S.RUNGE_KUTTA_FEHLBERG78, scn.epsilon, tl)[0] != 0:
+                break
+            steps += 1
+        ref.close()
+        systems += 1
+    dt = time.process_time() - t0
+    return dict(system_steps_per_s=steps / dt, cores=1, kind="reference",
+                sample=f"{steps} accepted RKF78 steps of {systems} systems (every 64th), {dt:.1f} s CPU")
 
 
 def adjoint_timing():
