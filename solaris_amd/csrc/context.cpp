@@ -510,7 +510,7 @@ int resident_run(sol_context *c, int ik, double eps, sol_timeline *tl, int64_t m
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(20000, (int64_t)(0.2 / per_step)));
     for (;;) {
         const int64_t k = max_steps > 0 ? std::min(chunk, max_steps - S.steps) : chunk;
-        launch_resident(c->stream, ik, c->res_d, 1, A, (int)k, block, c->n);
+        launch_resident(c->stream, ik, c->res_d, 1, A, (int)k, block, c->n, c->lds_limit);
         HIPCHECK(c, hipGetLastError());
         HIPCHECK(c, hipMemcpyAsync(c->res_h, c->res_d, sizeof(ResSystem), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));

@@ -141,7 +141,7 @@ int launch_round(sol_ensemble *e, int ik, int steps)
             ECHECK(e, hipEventCreate(&t0));
             ECHECK(e, hipEventCreate(&t1));
             (void)hipEventRecord(t0, e->stream);
-            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, block, b.nmax);
+            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, block, b.nmax, 0);
             ECHECK(e, hipGetLastError());
             (void)hipEventRecord(t1, e->stream);
             (void)hipEventSynchronize(t1);
@@ -151,7 +151,7 @@ int launch_round(sol_ensemble *e, int ik, int steps)
             (void)hipEventDestroy(t1);
             if (b.best_ms <= 0.0 || ms < b.best_ms) { b.best_ms = ms; b.block = block; }
         } else {
-            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, b.block, b.nmax);
+            launch_resident(e->stream, ik, e->sys_d + b.first, b.count, A, steps, b.block, b.nmax, 0);
             ECHECK(e, hipGetLastError());
         }
     }

@@ -40,7 +40,9 @@ struct ResArrays {
 
 int resident_block_size(int n);
 size_t resident_lds_bytes(int nmax);
+// lds_limit: LDS bytes a workgroup may use for the optional LDS-held stage
+// derivatives (0: none; the ensemble passes 0 to keep its occupancy).
 void launch_resident(hipStream_t st, int integ, ResSystem *systems, int nsys, const ResArrays &A, int max_steps,
-                     int block, int nmax);
+                     int block, int nmax, size_t lds_limit);
 
 }  // namespace sol

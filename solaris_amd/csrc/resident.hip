@@ -124,6 +124,28 @@ constexpr double dcoef(int s, int l) { return INTEG == INTEG_RKF78 ? tab::kRkfD[
 // Own bodies of this thread: i = threadIdx.x + q * blockDim.x, q < BPT.
 __device__ __forceinline__ int own(int q) { return threadIdx.x + q * blockDim.x; }
 
+// With one body per thread the state y0 and the previous stage's
+// derivative stay in registers (Reg); with more, both are read back from
+// memory.
+//
+// LF (RKF78, one body per thread, a single system whose LDS allows it):
+// the four most-read stage derivatives, F0, F3, F4 and F5 (32 of the 53
+// reads per component of an attempt), are also kept in LDS, each thread's
+// in its own row of kLdsStride doubles, so that every slot and component
+// is a fixed offset from one address.  The stride is odd, so a wave's
+// 8-byte reads take the two LDS passes they need and no more.
+template <int BPT, bool LF = false>
+struct Reg {
+    static constexpr bool on = BPT == 1;
+    static constexpr bool lds_f = LF;
+    double y0[BPT][6], prev[BPT][6];
+    double *sF;   // LF: this thread's LDS row
+};
+
+constexpr int kLdsSlots = 4, kLdsStride = 6 * kLdsSlots + 1;
+// LDS slot of stage derivative L (-1: memory only)
+__host__ __device__ constexpr int lds_slot(int L) { return L == 0 ? 0 : L == 3 ? 1 : L == 4 ? 2 : L == 5 ? 3 : -1; }
+
 // Publishes the stage positions of the own bodies to LDS: sJ[i] = (x, y, z,
 // k^2 m_i), and for the bodies that are sources of the indirect term (AC,
 // 1 <= i < end_max) sT[i] = r_i rm3_i.  rm3 = 1 / |r_i|^3 of the own bodies
@@ -225,9 +247,9 @@ __device__ void gravity_one(const Ctx &c, int i, double mu, double rm3, double &
 // vel, zero for the astrocentric star; acceleration half from gravity); the
 // last stage also writes the side outputs.  Ends with the barrier after
 // which sJ and sT may be overwritten.
-template <int BPT, bool BC, bool LAST>
-__device__ void evaluate_t(const Ctx &c, int l, const double (&vel)[BPT][3], const double (&mu)[BPT],
-                           const double (&rm3)[BPT], double (&f)[BPT][6])
+template <int BPT, bool BC, bool LAST, bool LF>
+__device__ void evaluate_t(const Ctx &c, const Reg<BPT, LF> &R, int l, const double (&vel)[BPT][3],
+                           const double (&mu)[BPT], const double (&rm3)[BPT], double (&f)[BPT][6])
 {
 #pragma unroll
     for (int q = 0; q < BPT; ++q) {
@@ -241,11 +263,20 @@ __device__ void evaluate_t(const Ctx &c, int l, const double (&vel)[BPT][3], con
         f[q][1] = star_ac ? 0.0 : vel[q][1];
         f[q][2] = star_ac ? 0.0 : vel[q][2];
         f[q][3] = ax; f[q][4] = ay; f[q][5] = az;
-        uint32_t vb[6];
-        offsets(c, i, vb);
-        double *fs = stage(c, l);
+        // LF: the LDS-held derivatives live only there, except F0, which the
+        // host reads as ACCEL
+        const int sl = LF ? lds_slot(l) : -1;
+        if (sl >= 0) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) st(fs, vb[k], f[q][k]);
+            for (int k = 0; k < 6; ++k) R.sF[sl * 6 + k] = f[q][k];
+        }
+        if (sl < 0 || l == 0) {
+            uint32_t vb[6];
+            offsets(c, i, vb);
+            double *fs = stage(c, l);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) st(fs, vb[k], f[q][k]);
+        }
         if (LAST) {
             if (!BC) c.A->rm3[c.base + i] = rm3[q];
             c.A->nni[c.base + i] = nni;
@@ -255,33 +286,26 @@ __device__ void evaluate_t(const Ctx &c, int l, const double (&vel)[BPT][3], con
     __syncthreads();
 }
 
-template <int BPT>
-__device__ __forceinline__ void evaluate(const Ctx &c, int l, bool last, const double (&vel)[BPT][3],
-                                         const double (&mu)[BPT], const double (&rm3)[BPT], double (&f)[BPT][6])
+template <int BPT, bool LF>
+__device__ __forceinline__ void evaluate(const Ctx &c, const Reg<BPT, LF> &R, int l, bool last,
+                                         const double (&vel)[BPT][3], const double (&mu)[BPT],
+                                         const double (&rm3)[BPT], double (&f)[BPT][6])
 {
     if (c.bc) {
-        if (last) evaluate_t<BPT, true, true>(c, l, vel, mu, rm3, f);
-        else evaluate_t<BPT, true, false>(c, l, vel, mu, rm3, f);
+        if (last) evaluate_t<BPT, true, true>(c, R, l, vel, mu, rm3, f);
+        else evaluate_t<BPT, true, false>(c, R, l, vel, mu, rm3, f);
     } else {
-        if (last) evaluate_t<BPT, false, true>(c, l, vel, mu, rm3, f);
-        else evaluate_t<BPT, false, false>(c, l, vel, mu, rm3, f);
+        if (last) evaluate_t<BPT, false, true>(c, R, l, vel, mu, rm3, f);
+        else evaluate_t<BPT, false, false>(c, R, l, vel, mu, rm3, f);
     }
 }
 
-// With one body per thread the state y0 and the previous stage's
-// derivative stay in registers (Reg); with more, both are read back from
-// memory.
-template <int BPT>
-struct Reg {
-    static constexpr bool on = BPT == 1;
-    double y0[BPT][6], prev[BPT][6];
-};
-
 // Stage derivative L of own body q, component k (at byte offset o).
-template <int L, int S, int BPT>
-__device__ __forceinline__ double fget(const Ctx &c, const Reg<BPT> &R, int q, int k, uint32_t o)
+template <int L, int S, int BPT, bool LF>
+__device__ __forceinline__ double fget(const Ctx &c, const Reg<BPT, LF> &R, int q, int k, uint32_t o)
 {
-    if constexpr (Reg<BPT>::on && L == S - 1 && L > 0) return R.prev[q][k];
+    if constexpr (Reg<BPT, LF>::on && L == S - 1 && L > 0) return R.prev[q][k];
+    else if constexpr (LF && lds_slot(L) >= 0) return R.sF[lds_slot(L) * 6 + k];
     else return ld(stage(c, L), o);
 }
 
@@ -296,8 +320,8 @@ __device__ __forceinline__ double fget(const Ctx &c, const Reg<BPT> &R, int q, i
 // Dormand-Prince RKN (DormandPrince.cpp:246-491): positions
 // p0 + c_S h v0 + h^2 var and velocities v0 + h var, var = sum_l a_{S,l} F_l
 // over the acceleration halves.
-template <int INTEG, int S, int BPT>
-__device__ __forceinline__ void stage_state(const Ctx &c, const Reg<BPT> &R, double h, double (&pos)[BPT][3],
+template <int INTEG, int S, int BPT, bool LF>
+__device__ __forceinline__ void stage_state(const Ctx &c, const Reg<BPT, LF> &R, double h, double (&pos)[BPT][3],
                                             double (&vel)[BPT][3])
 {
 #pragma unroll
@@ -363,21 +387,21 @@ __device__ __forceinline__ void stage_state(const Ctx &c, const Reg<BPT> &R, dou
 }
 
 // stage_state for the run-time stage number s (1 <= s <= n_stages).
-template <int INTEG, int BPT, int S = 1>
-__device__ __forceinline__ void stage_state_at(int s, const Ctx &c, const Reg<BPT> &R, double h,
+template <int INTEG, int BPT, bool LF, int S = 1>
+__device__ __forceinline__ void stage_state_at(int s, const Ctx &c, const Reg<BPT, LF> &R, double h,
                                                double (&pos)[BPT][3], double (&vel)[BPT][3])
 {
     if constexpr (S <= n_stages<INTEG>()) {
         if (s == S) stage_state<INTEG, S, BPT>(c, R, h, pos, vel);
-        else stage_state_at<INTEG, BPT, S + 1>(s, c, R, h, pos, vel);
+        else stage_state_at<INTEG, BPT, LF, S + 1>(s, c, R, h, pos, vel);
     }
 }
 
 // Final state into Y (and into ynew) and the error estimate into ERR for
 // the own bodies; returns this thread's contribution to the error maximum
 // (RKF78: max |err / yscale|, Dormand-Prince: max position error).
-template <int INTEG, int BPT>
-__device__ double final_state(const Ctx &c, const Reg<BPT> &R, double h, double (&ynew)[BPT][6])
+template <int INTEG, int BPT, bool LF>
+__device__ double final_state(const Ctx &c, const Reg<BPT, LF> &R, double h, double (&ynew)[BPT][6])
 {
     constexpr int NS = n_stages<INTEG>() + 1;
     double ratio = 0.0;
@@ -462,8 +486,9 @@ __device__ double final_state(const Ctx &c, const Reg<BPT> &R, double h, double 
 // One workgroup per system; BPT bodies per thread at most
 // (n <= BPT * blockDim.x).  Blocks have at most 640 threads with one body
 // per thread (168 registers) and 512 with more (256 registers).
-// Dynamic LDS: sJ and sT, nmax records each.
-template <int INTEG, int BPT>
+// Dynamic LDS: sJ and sT, nmax records each, and with LF the rows of the
+// LDS-held stage derivatives (nmax x kLdsStride doubles).
+template <int INTEG, int BPT, bool LF>
 __global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *systems, ResArrays A, int max_steps,
                                                                      int nmax)
 {
@@ -488,7 +513,8 @@ __global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *sy
         gm[q] = kGauss2 * A.mass[c.base + i];
         mu[q] = kGauss2 * (A.mass[c.base] + A.mass[c.base + i]);
     }
-    Reg<BPT> R;
+    Reg<BPT, LF> R;
+    R.sF = LF ? reinterpret_cast<double *>(lds + 2 * nmax) + own(0) * kLdsStride : nullptr;
     if constexpr (kReg) {
 #pragma unroll
         for (int q = 0; q < BPT; ++q) {
@@ -520,7 +546,7 @@ __global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *sy
                 for (int k = 0; k < 3; ++k) { pos[q][k] = y0[q][k]; vel[q][k] = y0[q][k + 3]; }
             }
             publish<BPT>(c, pos, gm, rm3);
-            evaluate<BPT>(c, 0, false, vel, mu, rm3, f);
+            evaluate<BPT>(c, R, 0, false, vel, mu, rm3, f);
             if (INTEG == INTEG_RKF78) {
 #pragma unroll
                 for (int q = 0; q < BPT; ++q) {
@@ -544,7 +570,7 @@ __global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *sy
                 double pos[BPT][3], vel[BPT][3], rm3[BPT], f[BPT][6];
                 stage_state_at<INTEG, BPT>(s, c, R, h, pos, vel);
                 publish<BPT>(c, pos, gm, rm3);
-                evaluate<BPT>(c, s, s == nst, vel, mu, rm3, f);
+                evaluate<BPT>(c, R, s, s == nst, vel, mu, rm3, f);
                 if constexpr (kReg) {
 #pragma unroll
                     for (int q = 0; q < BPT; ++q)
@@ -697,14 +723,32 @@ size_t resident_lds_bytes(int nmax) { return 2 * (size_t)nmax * sizeof(double4);
 
 namespace {
 
+size_t lds_f_bytes(int nmax) { return (size_t)nmax * kLdsStride * sizeof(double); }
+
 template <int INTEG>
 void launch_bpt(hipStream_t st, ResSystem *systems, int nsys, const ResArrays &A, int max_steps, int block,
-                int bpt, int nmax, size_t lds)
+                int bpt, int nmax, size_t lds, bool lf)
 {
+    if constexpr (INTEG == INTEG_RKF78) {
+        if (bpt == 1 && lf) {
+            hipLaunchKernelGGL((k_resident<INTEG, 1, true>), dim3(nsys), dim3(block), lds + lds_f_bytes(nmax), st,
+                               systems, A, max_steps, nmax);
+            return;
+        }
+    }
     switch (bpt) {
-    case 1: hipLaunchKernelGGL((k_resident<INTEG, 1>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps, nmax); break;
-    case 2: hipLaunchKernelGGL((k_resident<INTEG, 2>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps, nmax); break;
-    default: hipLaunchKernelGGL((k_resident<INTEG, 4>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps, nmax); break;
+    case 1:
+        hipLaunchKernelGGL((k_resident<INTEG, 1, false>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps,
+                           nmax);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_resident<INTEG, 2, false>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps,
+                           nmax);
+        break;
+    default:
+        hipLaunchKernelGGL((k_resident<INTEG, 4, false>), dim3(nsys), dim3(block), lds, st, systems, A, max_steps,
+                           nmax);
+        break;
     }
 }
 
@@ -713,18 +757,21 @@ void launch_bpt(hipStream_t st, ResSystem *systems, int nsys, const ResArrays &A
 // Bodies per thread: the smallest of 1, 2, 4 that covers nmax.  A block of
 // more than 640 threads, or of more than 512 that does not cover nmax, is
 // narrowed to 512; one too small for 4 per thread is widened to the
-// smallest that is not.
+// smallest that is not.  With lds_limit > 0 (a single system), RKF78 at
+// one body per thread keeps stage derivatives in LDS when they fit in
+// lds_limit bytes.
 void launch_resident(hipStream_t st, int integ, ResSystem *systems, int nsys, const ResArrays &A, int max_steps,
-                     int block, int nmax)
+                     int block, int nmax, size_t lds_limit)
 {
     const size_t lds = resident_lds_bytes(nmax);
     if (block > 640 || (block > 512 && nmax > block)) block = 512;
     if (block * 4 < nmax) block = ((nmax + 3) / 4 + kWave - 1) / kWave * kWave;
     const int bpt = nmax <= block ? 1 : nmax <= 2 * block ? 2 : 4;
+    const bool lf = lds_limit > 0 && lds + lds_f_bytes(nmax) <= lds_limit;
     switch (integ) {
-    case INTEG_RKF78: launch_bpt<INTEG_RKF78>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds); break;
-    case INTEG_RK4: launch_bpt<INTEG_RK4>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds); break;
-    default: launch_bpt<INTEG_DP>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds); break;
+    case INTEG_RKF78: launch_bpt<INTEG_RKF78>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds, lf); break;
+    case INTEG_RK4: launch_bpt<INTEG_RK4>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds, false); break;
+    default: launch_bpt<INTEG_DP>(st, systems, nsys, A, max_steps, block, bpt, nmax, lds, false); break;
     }
 }
 
