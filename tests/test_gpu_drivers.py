@@ -166,3 +166,14 @@ def test_integrate_clips_to_length_and_output(path):
     np.testing.assert_allclose(ctx.get_array(runtime.Y0)[:, :3], orc.get_array(runtime.Y0)[:, :3],
                                rtol=0, atol=1e-8)
     np.testing.assert_allclose(ctx.integrals(), orc.integrals(), rtol=1e-9, atol=1e-20)
+
+
+@pytest.mark.parametrize("integrator", [S.RUNGE_KUTTA_FEHLBERG78, S.DORMAND_PRINCE])
+def test_resident_four_bodies_per_thread(integrator):
+    """A single system of 1,500 bodies (star, 8 planets, 1,491 test
+    particles) runs the workgroup-resident driver with four bodies per
+    thread (384 threads); replayed against the oracle."""
+    scn = S.make_c5(n_tp=1491)
+    assert scn.n == 1500
+    print(f"[resident, 4 bodies per thread, integrator {integrator}]", flush=True)
+    replay(scn, integrator, 1e-10, 10, 1.0)

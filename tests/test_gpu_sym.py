@@ -18,6 +18,7 @@ import os
 import numpy as np
 import pytest
 
+from invariants import drifts
 from oracle.cpu_ref import Oracle
 from solaris_amd import _abi, runtime, scenarios as S
 
@@ -174,3 +175,38 @@ def test_sym_mixed_close_pairs_in_every_position():
     err = rel_err(a_m[:, 3:], b[:, 3:])
     print(f"  close pairs {pairs}: max rel err {err.max():.2e} at body {int(err.argmax())}", flush=True)
     assert err.max() <= 1e-7
+
+
+@pytest.mark.parametrize("precision", [runtime.FP64, runtime.MIXED])
+def test_c3_subset_drift_against_oracle(precision):
+    """Energy and angular-momentum drift through the symmetric path (C3
+    disk, 2,601 bodies, softening 1e-5 AU, RKF78 1e-10 for 30 days through
+    the caller loop) against the OpenMP oracle over the same span.  FP64
+    must drift no more than the oracle (twice its drift plus 1e-13);
+    the mixed form's drift is reported and bounded by ten times the
+    oracle's plus 1e-11 (its FP32 pair terms carry ~1e-7 of the
+    self-gravity, itself ~1e-5 of the total)."""
+    scn = S.make_c3(n=2600)
+    ctx = runtime.Context(scn, precision=precision)
+    orc = Oracle(scn, lib="omp")
+    T = 30.0
+    ctx.set_state(scn.y0)
+    orc.set_state(scn.y0)
+    tg = _abi.timeline(0.0, scn.h_next, T)
+    to = _abi.timeline(0.0, scn.h_next, T)
+    rg, sg = ctx.integrate(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, tg)
+    ro, so = orc.integrate(S.RUNGE_KUTTA_FEHLBERG78, 1e-10, to)
+    assert rg == ro == 0 and tg.time == to.time == T, (rg, ro, tg.time, to.time)
+    yg, yo = ctx.get_array(runtime.Y0), orc.get_array(runtime.Y0)
+    dEg, dLg = drifts(scn.y0, yg, scn.mass)
+    dEo, dLo = drifts(scn.y0, yo, scn.mass)
+    d = np.abs(yg[:, :3] - yo[:, :3]).max()
+    print(f"  [{'mixed' if precision == runtime.MIXED else 'fp64'}] {sg.accepted_steps} vs {so.accepted_steps} steps; "
+          f"max |dr| {d:.2e} AU; energy drift GPU {dEg:.2e} oracle {dEo:.2e}; AM drift GPU {dLg:.2e} "
+          f"oracle {dLo:.2e}", flush=True)
+    if precision == runtime.FP64:
+        assert dEg <= 2 * dEo + 1e-13
+        assert dLg <= 2 * dLo + 1e-13
+    else:
+        assert dEg <= 10 * dEo + 1e-11
+        assert dLg <= 10 * dLo + 1e-11
