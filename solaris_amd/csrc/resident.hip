@@ -12,12 +12,15 @@
 // stops with its own status and the others go on.
 //
 // Within a system each thread owns bodies i = tid, tid + blockDim, ...
-// The positions of the stage being evaluated are staged in LDS as
-// (x, y, z, k^2 m) records; every sink reads them as a broadcast stream.
+// (one, two or four of them).  The positions of the stage being evaluated
+// are staged in LDS as (x, y, z, k^2 m) records, with r_j rm3_j records for
+// the indirect term's sources; every sink reads them as a broadcast stream.
 // Stage derivatives live in global memory ([stage][6][TOT] over all
 // systems' bodies, TOT = sum of per-system padded counts) and are read
-// back only by the thread that wrote them.  Gravity only (no gas): systems
-// with a nebula use the multi-kernel path.
+// back only by the thread that wrote them; y0 and the previous stage stay
+// in registers, and for a single RKF78 system four derivatives stay in LDS
+// too.  Gravity only (no gas): systems with a nebula use the multi-kernel
+// path.
 #include <float.h>
 
 #include <type_traits>
@@ -28,13 +31,6 @@
 
 namespace sol {
 namespace {
-
-__device__ __forceinline__ double wsum(double v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
 
 __device__ __forceinline__ double wmax(double v)
 {
