@@ -40,6 +40,8 @@ struct AdjointState {
     double *mbar = nullptr, *m0blk = nullptr, *mout = nullptr;
     double *psink = nullptr, *psrc = nullptr;
     int psink_cap = 0, psrc_cap = 0;   // chunks allocated
+    double *hd = nullptr;              // step sizes on the device (resident path)
+    size_t hd_cap = 0;
     VjpArgs shape{};                   // launch shape of the products
     std::vector<void *> allocs;
 };
@@ -52,6 +54,7 @@ void adjoint_release(sol_context *c)
     if (c->adj->seg) (void)hipFree(c->adj->seg);
     if (c->adj->psink) (void)hipFree(c->adj->psink);
     if (c->adj->psrc) (void)hipFree(c->adj->psrc);
+    if (c->adj->hd) (void)hipFree(c->adj->hd);
     delete c->adj;
     c->adj = nullptr;
 }
@@ -212,12 +215,30 @@ int sol_adjoint_forward(sol_context *c, double t0, double h, const double *h_seq
     const size_t v = 6 * (size_t)c->np;
     HIPCHECK(c, hipMemcpyAsync(A->ya, c->y0, v * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     double *cur = A->ya, *nxt = A->yb;
-    for (int64_t i = 0; i < nsteps; ++i) {
-        if (i % K == 0)
-            HIPCHECK(c, hipMemcpyAsync(A->ckpt + (size_t)(i / K) * v, cur, v * sizeof(double),
+    if (resident_rk4_ok(c)) {
+        // one launch of the workgroup-resident driver per segment
+        if (A->hd_cap < (size_t)nsteps) {
+            if (A->hd) (void)hipFree(A->hd);
+            A->hd = nullptr;
+            A->hd_cap = 0;
+            HIPCHECK(c, hipMalloc(&A->hd, nsteps * sizeof(double)));
+            A->hd_cap = (size_t)nsteps;
+        }
+        HIPCHECK(c, hipMemcpyAsync(A->hd, hs.data(), nsteps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        for (int64_t n0 = 0; n0 < nsteps; n0 += K) {
+            const int64_t n1 = std::min<int64_t>(nsteps, n0 + K);
+            HIPCHECK(c, hipMemcpyAsync(A->ckpt + (size_t)(n0 / K) * v, cur, v * sizeof(double),
                                        hipMemcpyDeviceToDevice, c->stream));
-        if ((r = rk4_step(c, A, cur, ts[i], hs[i], nullptr, nxt))) return r;
-        std::swap(cur, nxt);
+            if ((r = resident_rk4(c, cur, nxt, ts[n0], A->hd + n0, n1 - n0, nullptr))) return r;
+        }
+    } else {
+        for (int64_t i = 0; i < nsteps; ++i) {
+            if (i % K == 0)
+                HIPCHECK(c, hipMemcpyAsync(A->ckpt + (size_t)(i / K) * v, cur, v * sizeof(double),
+                                           hipMemcpyDeviceToDevice, c->stream));
+            if ((r = rk4_step(c, A, cur, ts[i], hs[i], nullptr, nxt))) return r;
+            std::swap(cur, nxt);
+        }
     }
     HIPCHECK(c, hipMemcpyAsync(c->y, cur, v * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     if (y_final) {
@@ -262,13 +283,19 @@ int sol_adjoint_backward(sol_context *c, const double *grad_y_final, double *gra
     for (int64_t sg = (int64_t)A->ncp - 1; sg >= 0; --sg) {
         const int64_t n0 = sg * A->K, n1 = std::min<int64_t>(A->nsteps, n0 + A->K);
         // recompute the segment's stage states from its checkpoint
-        const double *cur = A->ckpt + (size_t)sg * v;
-        double *bufs[2] = {A->ya, A->yb};
-        for (int64_t i = n0; i < n1; ++i) {
-            double *stages = A->seg + (size_t)(i - n0) * 4 * v;
-            double *nxt = (i + 1 < n1) ? bufs[(i - n0) & 1] : nullptr;
-            if ((r = rk4_step(c, A, cur, A->t[i], A->h[i], stages, nxt))) return r;
-            cur = nxt;
+        if (resident_rk4_ok(c) && A->hd) {
+            HIPCHECK(c, hipMemcpyAsync(A->ya, A->ckpt + (size_t)sg * v, v * sizeof(double), hipMemcpyDeviceToDevice,
+                                       c->stream));
+            if ((r = resident_rk4(c, A->ya, A->yb, A->t[n0], A->hd + n0, n1 - n0, A->seg))) return r;
+        } else {
+            const double *cur = A->ckpt + (size_t)sg * v;
+            double *bufs[2] = {A->ya, A->yb};
+            for (int64_t i = n0; i < n1; ++i) {
+                double *stages = A->seg + (size_t)(i - n0) * 4 * v;
+                double *nxt = (i + 1 < n1) ? bufs[(i - n0) & 1] : nullptr;
+                if ((r = rk4_step(c, A, cur, A->t[i], A->h[i], stages, nxt))) return r;
+                cur = nxt;
+            }
         }
         for (int64_t i = n1 - 1; i >= n0; --i) {
             const double h = A->h[i];

@@ -633,6 +633,42 @@ int sol::eval_force(sol_context *c, const double *st, double t, double *out)
     return evaluate_at(c, st, t, 0.0, out, EPI_COMPUTE, INTEG_RK4, false, false);
 }
 
+bool sol::resident_rk4_ok(const sol_context *c) { return use_resident(c); }
+
+int sol::resident_rk4(sol_context *c, double *y, double *ytmp, double t0, const double *h_seq, int64_t nsteps,
+                      double *stages)
+{
+    if (nsteps <= 0) return SOL_OK;
+    if (nsteps > INT32_MAX) return c->fail(SOL_ERR_ARGUMENT, "resident RK4: too many steps for one launch");
+    ResSystem &S = *c->res_h;
+    S = ResSystem{};
+    S.n = c->n;
+    S.nm = c->n_massive;
+    source_ends(c, S.end_a, S.end_b);
+    S.base = 0;
+    S.barycentric = c->barycentric;
+    S.status = RES_RUNNING;
+    S.t = t0;
+    S.length = 1.0e300;
+    HIPCHECK(c, hipMemcpyAsync(c->res_d, c->res_h, sizeof(ResSystem), hipMemcpyHostToDevice, c->stream));
+    ResArrays A{};
+    A.tot = c->np;
+    A.y0 = y; A.y = ytmp; A.acc0 = c->acc0; A.yscale = c->yscale; A.err = c->error; A.fst = c->fst;
+    A.mass = c->mass; A.rm3 = c->rm3; A.nnd = c->nnd; A.nni = c->nni;
+    A.radius = c->radius;
+    A.h_seq = h_seq;
+    A.stages_out = stages;
+    launch_resident(c->stream, INTEG_RK4, c->res_d, 1, A, (int)nsteps, resident_block_size(c->n), c->n,
+                    c->lds_limit);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(c->res_h, c->res_d, sizeof(ResSystem), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (S.steps != nsteps)
+        return c->fail(SOL_ERR_DEVICE, "resident RK4: %lld of %lld steps done", (long long)S.steps,
+                       (long long)nsteps);
+    return SOL_OK;
+}
+
 extern "C" {
 
 const char *sol_version(void) { return "solaris_amd 0.1 (gfx950)"; }

@@ -122,6 +122,15 @@ void source_ends(const sol_context *c, int &ea, int &eb);
 // `st` (6 x NP) at time t into `out` (6 x NP), on the context's stream.
 int eval_force(sol_context *c, const double *st, double t, double *out);
 
+// Whether the workgroup-resident driver can run this context (small, no
+// gas), and nsteps fixed RK4 steps with it from the state y (6 x NP, updated
+// in place; ytmp is scratch of the same size) at t0 with the device step
+// sizes h_seq, writing the stage states [nsteps][4][6 NP] to stages when
+// it is not null.  One launch; synchronous.
+bool resident_rk4_ok(const sol_context *c);
+int resident_rk4(sol_context *c, double *y, double *ytmp, double t0, const double *h_seq, int64_t nsteps,
+                 double *stages);
+
 // Frees the adjoint's recorded state (adjoint.cpp).
 void adjoint_release(sol_context *c);
 // Marks the recorded forward pass stale (parameters or force model changed).

@@ -36,6 +36,11 @@ struct ResArrays {
     double *y0, *y, *acc0, *yscale, *err, *fst, *mass, *rm3, *nnd;
     int32_t *nni;
     const double *radius;     // read by the collision check only
+    // RK4 only, both optional: the step size of each step of the launch
+    // (h_seq[step], step counted from the launch's first), and the stage
+    // states Y1..Y4 of each step at stages_out + (step * 4 + s) * 6 * tot
+    const double *h_seq;
+    double *stages_out;
 };
 
 int resident_block_size(int n);

@@ -524,6 +524,9 @@ __global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *sy
     for (int step = 0; step < max_steps; ++step) {
         const double t0 = S.t;
         double h = S.h_next;
+        if constexpr (INTEG == INTEG_RK4) {
+            if (A.h_seq) h = A.h_seq[step];
+        }
         // f0 at y0 (the Driver's first evaluation), and yscale
         {
             double y0[BPT][6], pos[BPT][3], vel[BPT][3], rm3[BPT], f[BPT][6];
@@ -540,6 +543,15 @@ __global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *sy
                 }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) { pos[q][k] = y0[q][k]; vel[q][k] = y0[q][k + 3]; }
+                if constexpr (INTEG == INTEG_RK4) {
+                    if (A.stages_out && own(q) < c.n) {
+                        uint32_t vb[6];
+                        offsets(c, own(q), vb);
+                        double *Y = A.stages_out + (size_t)(step * 4) * 6 * c.tot;
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) st(Y, vb[k], y0[q][k]);
+                    }
+                }
             }
             publish<BPT>(c, pos, gm, rm3);
             evaluate<BPT>(c, R, 0, false, vel, mu, rm3, f);
@@ -565,6 +577,19 @@ __global__ __launch_bounds__(BPT == 1 ? 640 : 512) void k_resident(ResSystem *sy
             for (int s = 1; s <= nst; ++s) {
                 double pos[BPT][3], vel[BPT][3], rm3[BPT], f[BPT][6];
                 stage_state_at<INTEG, BPT>(s, c, R, h, pos, vel);
+                if constexpr (INTEG == INTEG_RK4) {
+                    if (A.stages_out) {
+#pragma unroll
+                        for (int q = 0; q < BPT; ++q) {
+                            if (own(q) >= c.n) break;
+                            uint32_t vb[6];
+                            offsets(c, own(q), vb);
+                            double *Y = A.stages_out + (size_t)(step * 4 + s) * 6 * c.tot;
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) { st(Y, vb[k], pos[q][k]); st(Y, vb[k + 3], vel[q][k]); }
+                        }
+                    }
+                }
                 publish<BPT>(c, pos, gm, rm3);
                 evaluate<BPT>(c, R, s, s == nst, vel, mu, rm3, f);
                 if constexpr (kReg) {
