@@ -168,42 +168,49 @@ __device__ void publish(const Ctx &c, const double (&pos)[BPT][3], const double 
     __syncthreads();
 }
 
-// Pair sums of sink p over the sources j in [j0, j1) (which exclude the
-// sink itself).  AC (GravityAC, Solaris/Acceleration.cpp:294-317): over
-// the sources in order, k^2 m_j (d_ij / r_ij^3 - r_j rm3_j), the indirect
-// term taken pair by pair as the reference does.  BC (GravityBC,
+// Pair sums of sink i (position p) over the sources j in [j0, j1), the
+// sink itself left out.  AC (GravityAC, Solaris/Acceleration.cpp:294-317):
+// over the sources in order, k^2 m_j (d_ij / r_ij^3 - r_j rm3_j), the
+// indirect term taken pair by pair as the reference does.  BC (GravityBC,
 // :530-634): k^2 m_j d_ij / r_ij^3.  1 / r_ij^3 is the hardware reciprocal
 // square root cubed with one second-order correction,
 // (1 - e)^(-3/2) ~ 1 + 3/2 e + 15/8 e^2 for e = 1 - r^2 y^2.  NN: the
 // nearest source (BC: the last of equals, AC: the first).
+// One loop over the whole range, the sink's own term masked rather than
+// split off (two ranges meant two divergent loops per sink, the second run
+// again for the sinks inside the source range): its weight is zero and its
+// r^2 is replaced by 1, so it adds exactly +-0 and the sums are those of
+// the loop that skips it.
 template <bool BC, bool NN>
-__device__ __forceinline__ void pair_range(const Ctx &c, const double4 &p, int j0, int j1, double &px, double &py,
-                                           double &pz, double &best, int &bidx)
+__device__ __forceinline__ void pair_range(const Ctx &c, const double4 &p, int i, int j0, int j1, double &px,
+                                           double &py, double &pz, double &best, int &bidx)
 {
 #pragma unroll 2
     for (int j = j0; j < j1; ++j) {
         const double4 q = c.sJ[j];
         double4 t;
         if constexpr (!BC) t = c.sT[j];
+        const bool self = j == i;
         const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r2 = self ? 1.0 : dx * dx + dy * dy + dz * dz;
+        const double w = self ? 0.0 : q.w;
         const double y0 = __builtin_amdgcn_rsq(r2);
         const double tt = y0 * y0;
         const double e = __builtin_fma(-r2, tt, 1.0);
         const double y3 = tt * y0;
         const double rijm3 = __builtin_fma(y3, e * __builtin_fma(1.875, e, 1.5), y3);
         if constexpr (BC) {
-            const double f = q.w * rijm3;
+            const double f = w * rijm3;
             px = __builtin_fma(f, dx, px);
             py = __builtin_fma(f, dy, py);
             pz = __builtin_fma(f, dz, pz);
         } else {
-            px = __builtin_fma(q.w, __builtin_fma(dx, rijm3, -t.x), px);
-            py = __builtin_fma(q.w, __builtin_fma(dy, rijm3, -t.y), py);
-            pz = __builtin_fma(q.w, __builtin_fma(dz, rijm3, -t.z), pz);
+            px = __builtin_fma(w, __builtin_fma(dx, rijm3, -t.x), px);
+            py = __builtin_fma(w, __builtin_fma(dy, rijm3, -t.y), py);
+            pz = __builtin_fma(w, __builtin_fma(dz, rijm3, -t.z), pz);
         }
         if constexpr (NN) {
-            if (BC ? (r2 <= best) : (r2 < best)) { best = r2; bidx = j; }
+            if (!self && (BC ? (r2 <= best) : (r2 < best))) { best = r2; bidx = j; }
         }
     }
 }
@@ -224,8 +231,7 @@ __device__ void gravity_one(const Ctx &c, int i, double mu, double rm3, double &
     const int jb = BC ? 0 : 1;
     int je = BC ? c.nm : (i < c.nm) ? c.end_a : c.end_b;
     if (!BC && i == 0) je = jb;   // the star is not integrated (AC)
-    pair_range<BC, NN>(c, p, jb, min(i, je), px, py, pz, best, bidx);
-    pair_range<BC, NN>(c, p, max(i + 1, jb), je, px, py, pz, best, bidx);
+    pair_range<BC, NN>(c, p, i, jb, je, px, py, pz, best, bidx);
     if (BC) {
         ax = px; ay = py; az = pz;
     } else if (i == 0) {
