@@ -27,6 +27,7 @@
 
 #include "sol_internal.h"
 #include "resident.h"
+#include "static_for.h"
 #include "tableaux.h"
 
 namespace sol {
@@ -98,16 +99,6 @@ __device__ __forceinline__ void offsets(const Ctx &c, int i, uint32_t vb[6])
     asm volatile("" : "+v"(i));
 #pragma unroll
     for (int k = 0; k < 6; ++k) vb[k] = (uint32_t)((k * c.tot + c.base + i) * 8);
-}
-
-// Compile-time loop: fn(std::integral_constant<int, L>) for L in [B, E).
-template <int B, int E, typename Fn>
-__device__ __forceinline__ void static_for(Fn &&fn)
-{
-    if constexpr (B < E) {
-        fn(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(fn);
-    }
 }
 
 template <int INTEG>

@@ -20,6 +20,8 @@
 #include <float.h>
 
 #include "sol_internal.h"
+#include "static_for.h"
+#include "tableaux.h"
 
 namespace sol {
 namespace {
@@ -377,42 +379,80 @@ __device__ __forceinline__ double fetch_f(const EpiArgs &a, int l, int c, int i,
     return a.fst[(int64_t)(l - 1) * 6 * a.np + c * a.np + i];
 }
 
-// State of stage sn (the reference's yTemp before the sn-th evaluation).
-__device__ void combine_stage(const EpiArgs &a, const double *coef, int sn, int i, int s_reg,
-                              const double f[6], double h, double out[6])
+// State of stage SN (the reference's yTemp before the SN-th evaluation)
+// with the tableau compiled in (tableaux.h), so each stage is straight-line
+// code over its nonzero coefficients and every derivative load can be
+// issued before the first use.  A runtime loop over the packed table put
+// one coefficient read and one derivative load behind each term.
+// RKF78 and RK4: acc = the first nonzero d_l F_l, then acc + d_l F_l over
+// the others in ascending l, out = y0 + h acc (RungeKuttaFehlberg78.cpp:
+// 100-116, RungeKutta4.cpp:74-110).  Dormand-Prince (DormandPrince.cpp:
+// 274-409): var = a_0 F_0 + the nonzero a_l F_l in ascending l over the
+// acceleration halves; positions y0 + c h v0 + h^2 var, velocities
+// v0 + h var.  FREG: stage SN - 1's derivative is f (registers), else
+// every derivative comes from memory.
+template <int INTEG, int SN, bool FREG>
+__device__ __forceinline__ void combine_ct(const EpiArgs &a, int i, const double f[6], double h, double out[6])
 {
     const int np = a.np;
-    if (a.integ == INTEG_DP) {
-        const double c = coef[kDpC + sn];
+    constexpr int s_reg = FREG ? SN - 1 : -1;
+    if constexpr (INTEG == INTEG_DP) {
+        constexpr double cc = tab::kDpC[SN];
         const double h2 = h * h;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            double var = coef[kDpA + sn * 8 + 0] * fetch_f(a, 0, j + 3, i, s_reg, f);
-            for (int l = 1; l < sn; ++l) {
-                const double d = coef[kDpA + sn * 8 + l];
-                if (d == 0.0) continue;
-                var += d * fetch_f(a, l, j + 3, i, s_reg, f);
-            }
+            double var = tab::kDpA[SN][0] * fetch_f(a, 0, j + 3, i, s_reg, f);
+            static_for<1, SN>([&](auto L) {
+                constexpr double d = tab::kDpA[SN][L];
+                if constexpr (d != 0.0) var += d * fetch_f(a, L, j + 3, i, s_reg, f);
+            });
             const double p0 = a.y0[j * np + i], v0 = a.y0[(j + 3) * np + i];
-            out[j] = p0 + c * h * v0 + h2 * (var);
+            out[j] = p0 + cc * h * v0 + h2 * (var);
             out[j + 3] = v0 + h * (var);
         }
-        return;
-    }
-    const double *D = coef + (a.integ == INTEG_RKF78 ? kRkfD + sn * 12 : kRk4A + sn * 3);
+    } else {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        double acc = 0.0;
-        bool first = true;
-        for (int l = 0; l < sn; ++l) {
-            const double d = D[l];
-            if (d == 0.0) continue;
-            const double v = d * fetch_f(a, l, c, i, s_reg, f);
-            acc = first ? v : acc + v;
-            first = false;
+        for (int c = 0; c < 6; ++c) {
+            double acc = 0.0;
+            bool first = true;
+            static_for<0, SN>([&](auto L) {
+                constexpr double d = INTEG == INTEG_RKF78 ? tab::kRkfD[SN][L] : tab::kRk4A[SN][L];
+                if constexpr (d != 0.0) {
+                    const double v = d * fetch_f(a, L, c, i, s_reg, f);
+                    acc = first ? v : acc + v;
+                    first = false;
+                }
+            });
+            out[c] = a.y0[c * np + i] + h * acc;
         }
-        out[c] = a.y0[c * np + i] + h * acc;
     }
+}
+
+// combine_ct for the run-time stage sn; s_reg is sn - 1 (that stage's
+// derivative in f) or, for the first stage of an attempt, -1.
+template <int INTEG, int NS>
+__device__ __forceinline__ void combine_dispatch(const EpiArgs &a, int sn, int i, int s_reg, const double f[6],
+                                                 double h, double out[6])
+{
+    static_for<1, NS + 1>([&](auto S) {
+        if (sn == S) {
+            if constexpr (S == 1) {
+                if (s_reg < 0) {
+                    combine_ct<INTEG, 1, false>(a, i, f, h, out);
+                    return;
+                }
+            }
+            combine_ct<INTEG, S, true>(a, i, f, h, out);
+        }
+    });
+}
+
+__device__ void combine_stage(const EpiArgs &a, int sn, int i, int s_reg, const double f[6], double h,
+                              double out[6])
+{
+    if (a.integ == INTEG_RKF78) combine_dispatch<INTEG_RKF78, 12>(a, sn, i, s_reg, f, h, out);
+    else if (a.integ == INTEG_RK4) combine_dispatch<INTEG_RK4, 3>(a, sn, i, s_reg, f, h, out);
+    else combine_dispatch<INTEG_DP, 8>(a, sn, i, s_reg, f, h, out);
 }
 
 // Writes the next stage's state for body i: J record (positions and k^2 m),
@@ -531,7 +571,7 @@ __global__ __launch_bounds__(256) void k_epilogue(EpiArgs a)
         double v[6] = {0, 0, 0, 0, 0, 0};
         if (valid) {
             double y[6];
-            combine_stage(a, coef, a.stage + 1, i, a.stage, f, h, y);
+            combine_stage(a, a.stage + 1, i, a.stage, f, h, y);
             emit_next(a, i, y, v);
         }
         if (!a.barycentric) {
@@ -610,7 +650,7 @@ __global__ __launch_bounds__(256) void k_combine_first(EpiArgs a)
     if (i < a.n) {
         double y[6];
         const double dummy[6] = {0, 0, 0, 0, 0, 0};
-        combine_stage(a, coef, 1, i, -1, dummy, h, y);
+        combine_stage(a, 1, i, -1, dummy, h, y);
         emit_next(a, i, y, v);
     }
     if (!a.barycentric) {
