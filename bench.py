@@ -337,6 +337,14 @@ def main():
         traffic_probe(args.n)
         return
 
+    # The result is the only line on standard output.  Everything else,
+    # including what compiled code prints (the reference's GasDragAC
+    # announces its drag regime with printf), goes to standard error:
+    # file descriptor 1 points there until the line is written.
+    sys.stdout.flush()
+    result_fd = os.dup(1)
+    os.dup2(2, 1)
+
     from solaris_amd import runtime, scenarios as S
     scn = S.make_c3(n=args.n, softening=0.0)
     print(f"bench: star + {args.n} bodies, RKF78 1e-10, FP64, warmup {args.warmup}, steps {args.steps}",
@@ -426,7 +434,9 @@ def main():
             if allc:
                 allc["speedup_gpu_over_cpu"] = value / allc["value"]
                 line["cpu_all_cores"] = allc
-    print(json.dumps(line), flush=True)
+    sys.stdout.flush()
+    os.write(result_fd, (json.dumps(line) + "\n").encode())
+    os.close(result_fd)
 
 
 if __name__ == "__main__":
