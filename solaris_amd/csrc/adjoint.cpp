@@ -19,6 +19,7 @@
 // checkpoints (ceil(nsteps/K) x 6 NP doubles) and one segment of stage
 // states (4 K x 6 NP doubles).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "context.h"
@@ -82,7 +83,8 @@ int amalloc(sol_context *c, AdjointState *A, double **p, size_t count)
 
 // Launch shape of the vector-Jacobian products: sinks [1, n) in tiles of 64,
 // sources [1, jmax); the other dimension is cut into chunks so that a launch
-// has about 8 waves per CU.
+// has about 8 waves per CU.  With at most 64 sources the source sums go to
+// k_vjp_src_few instead (SOLARIS_AMD_VJP_WIDE=1 keeps k_vjp_src).
 void vjp_shape(const sol_context *c, VjpArgs &v)
 {
     v = VjpArgs{};
@@ -99,9 +101,19 @@ void vjp_shape(const sol_context *c, VjpArgs &v)
         v.ch_s = (nsrc + v.nch_s - 1) / v.nch_s;
         v.nch_s = (nsrc + v.ch_s - 1) / v.ch_s;
         v.tiles_b = (nsrc + kWave - 1) / kWave;
-        v.nch_b = std::max(1, std::min((waves + v.tiles_b - 1) / v.tiles_b, (nsink + kWave - 1) / kWave));
-        v.ch_b = (nsink + v.nch_b - 1) / v.nch_b;
-        v.nch_b = (nsink + v.ch_b - 1) / v.ch_b;
+        const char *wide = getenv("SOLARIS_AMD_VJP_WIDE");
+        if (nsrc <= kWave && !(wide && wide[0] == '1')) {
+            // k_vjp_src_few: lanes over sinks, about 8 chunks (each of a
+            // whole number of sinks per lane) so that k_vjp_combine has few
+            // partials to add per source
+            v.src_few = 1;
+            v.ch_b = std::max(1, (nsink + 8 * kWave - 1) / (8 * kWave)) * kWave;
+            v.nch_b = (nsink + v.ch_b - 1) / v.ch_b;
+        } else {
+            v.nch_b = std::max(1, std::min((waves + v.tiles_b - 1) / v.tiles_b, (nsink + kWave - 1) / kWave));
+            v.ch_b = (nsink + v.nch_b - 1) / v.nch_b;
+            v.nch_b = (nsink + v.ch_b - 1) / v.ch_b;
+        }
     }
 }
 

@@ -116,6 +116,53 @@ __global__ __launch_bounds__(64) void k_vjp_src(VjpArgs a)
     }
 }
 
+// The same source sums when the sources fit one wave (at most 64: a few
+// massive bodies and many test particles).  k_vjp_src would then run one
+// wave per chunk with a lane per source, a few lanes live, each looping
+// serially over the chunk's sinks.  Here the lanes take the sinks of chunk
+// blockIdx.x (ch_b of them, a multiple of 64, lane l every 64th from l),
+// the loop runs over the sources, and each source's seven sums are reduced
+// over the wave by a fixed butterfly.  The pair terms are k_vjp_src's;
+// psrc has the same layout.
+__global__ __launch_bounds__(64) void k_vjp_src_few(VjpArgs a)
+{
+    const int np = a.np;
+    const int lane = (int)threadIdx.x;
+    const int is = 1 + (int)blockIdx.x * a.ch_b;
+    const int ie = min(is + a.ch_b, a.n);
+    double *p = a.psrc + (size_t)blockIdx.x * 7 * np;
+    for (int k = 1; k < a.jmax; ++k) {
+        const double xk = a.Y[k], yk = a.Y[np + k], zk = a.Y[2 * np + k];
+        double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = is + lane; i < ie; i += kWave) {
+            const bool ok = k < sink_end(a, i) && i != k;
+            const double ux = a.W[3 * np + i], uy = a.W[4 * np + i], uz = a.W[5 * np + i];
+            const double dx = xk - a.Y[i], dy = yk - a.Y[np + i], dz = zk - a.Y[2 * np + i];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double rinv = ok ? 1.0 / sqrt(r2) : 0.0;
+            const double rinv2 = rinv * rinv;
+            const double rinv3 = rinv * rinv2;
+            const double s = dx * ux + dy * uy + dz * uz;
+            const double c2 = 3.0 * rinv3 * rinv2 * s;
+            v[0] += rinv3 * ux - c2 * dx;
+            v[1] += rinv3 * uy - c2 * dy;
+            v[2] += rinv3 * uz - c2 * dz;
+            v[3] += s * rinv3;
+            const double m = ok ? 1.0 : 0.0;
+            v[4] += m * ux;
+            v[5] += m * uy;
+            v[6] += m * uz;
+        }
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+#pragma unroll
+            for (int o = kWave / 2; o >= 1; o >>= 1) v[c] += __shfl_xor(v[c], o, kWave);
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < 7; ++c) p[(size_t)c * np + k] = v[c];
+    }
+}
+
 // Per body: G = VJP(W) from the partials, ybar += G, wnext = alpha*lam +
 // beta*G (when wnext is set), mass gradients accumulated (dL/dgm).
 __global__ __launch_bounds__(kBlock) void k_vjp_combine(VjpArgs a)
@@ -244,8 +291,12 @@ void launch_vjp(hipStream_t st, const VjpArgs &a)
 {
     if (a.tiles_s > 0 && a.nch_s > 0)
         hipLaunchKernelGGL(k_vjp_sink, dim3(a.tiles_s, a.nch_s), dim3(kWave), 0, st, a);
-    if (a.tiles_b > 0 && a.nch_b > 0)
-        hipLaunchKernelGGL(k_vjp_src, dim3(a.tiles_b, a.nch_b), dim3(kWave), 0, st, a);
+    if (a.tiles_b > 0 && a.nch_b > 0) {
+        if (a.src_few)
+            hipLaunchKernelGGL(k_vjp_src_few, dim3(a.nch_b), dim3(kWave), 0, st, a);
+        else
+            hipLaunchKernelGGL(k_vjp_src, dim3(a.tiles_b, a.nch_b), dim3(kWave), 0, st, a);
+    }
     hipLaunchKernelGGL(k_vjp_combine, dim3(a.np / kBlock), dim3(kBlock), 0, st, a);
 }
 

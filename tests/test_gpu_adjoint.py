@@ -161,6 +161,28 @@ def test_checkpoint_interval_does_not_change_gradient(c5):
         assert np.array_equal(gm, out[0][1])
 
 
+def test_source_sum_kernels_agree(c5, monkeypatch):
+    """C5's 8 sources take k_vjp_src_few (lanes over sinks, a wave
+    reduction per source); SOLARIS_AMD_VJP_WIDE=1 forces k_vjp_src (lanes
+    over sources, a serial loop over sinks).  The pair terms are the same
+    and only the summation order differs."""
+    steps = 200
+    out = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("SOLARIS_AMD_VJP_WIDE", wide)
+        ctx = runtime.Context(c5)
+        ctx.set_state(c5.y0)
+        yT = ctx.adjoint_forward(0.0, H, steps, K)
+        _, g = loss_and_grad(yT, yT[:, :3] + c5.target_offsets)
+        out.append(ctx.adjoint_backward(g))
+        ctx.close()
+    for few, wide in zip(out[0], out[1]):
+        scale = np.abs(wide).max()
+        diff = np.abs(few - wide).max()
+        print(f"max |few - wide| {diff:.3e} (scale {scale:.3e})")
+        assert diff <= 1e-12 * scale
+
+
 def test_step_sequence_is_replayed(c5):
     hs = 0.5 + (np.arange(300) % 4) * 0.25
     ctx = runtime.Context(c5)
